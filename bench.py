@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the blocked SPD solver path on one MI355X.
+
+Step (timed K times after W warm-ups) = the reference's Cholesky solve path
+on the N = 65536 GP system, inputs resident in HBM:
+  RBF assembly of K (N x N, packed lower block triangle, b = 256) from X
+  -> FP64 blocked Cholesky K = L L^T -> forward + backward triangular solves.
+value = (N^3/3) / step time in TFLOP/s (whole step, assembly and solves
+included). After the timed region, untimed checks and side measurements:
+backward error of the solution, per-phase kernel times of the factorisation
+(HIP events, for the roofline of the dominant kernel), the CG workload
+(N = 32768, b = 128) with its mat-vec bandwidth, and the CPU baseline (the
+oracle's restatement of the reference's CPU path on a bounded sample).
+
+Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "heterogeneous-solvers_amd")
+
+METRIC = "Cholesky N=65536 solve time+TFLOP/s: FP64 vs mixed+IR at equal residual; CG GB/s"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def packed_len(N, b):
+    n = -(-N // b)
+    return n * (n + 1) // 2 * b * b
+
+
+def cholesky_workload(hs, dev, syn, args):
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
+    if args.N:
+        N = args.N
+    n = -(-N // b)
+    X, y = syn.gp_inputs(seed, N, d)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    dX = dev.upload(X.reshape(-1))
+    dy = dev.upload(rhs)
+    dx = dev.zeros(n * b)
+    plen = packed_len(N, b)
+    log("cholesky workload N=%d b=%d d=%d: packed K %.2f GB" % (N, b, d, plen * 8 / 1e9))
+    dA = dev.alloc(plen * 8)
+    timer = hs.Timer(dev)
+    phase = (C.c_double * 3)()
+    launches = (C.c_int64 * 3)()
+    step_ms, phases = [], []
+    for s in range(args.warmup + args.steps):
+        timer.start()
+        hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+        info = hs.call("hs_cholesky_f64_timed", dev.h, dA.p, N, b, phase, launches)
+        hs.call("hs_memcpy_dtod", dev.h, dx.p, dy.p, n * b * 8)
+        hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+        timer.stop()
+        ms = timer.ms()
+        if info != 0:
+            raise RuntimeError("factorisation reported non-positive pivot %d" % info)
+        if s >= args.warmup:
+            step_ms.append(ms)
+            phases.append(list(phase))
+        log("  step %d: %.1f ms (potrf %.1f trsm %.1f update %.1f ms)" % (s, ms, phase[0], phase[1], phase[2]))
+    x = dev.download(dx, n * b)
+    # untimed check: residual on a freshly assembled K
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+    dr = dev.zeros(n * b)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dr.p, N, b)
+    Kx = dev.download(dr, n * b)
+    ones = np.zeros(n * b)
+    ones[:N] = 1.0
+    d1 = dev.upload(ones)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, d1.p, dr.p, N, b)
+    knorm_inf = float(np.max(dev.download(dr, n * b)))  # RBF entries are >= 0
+    res = rhs - Kx
+    rel_res = float(np.linalg.norm(res) / np.linalg.norm(rhs))
+    bwd = float(np.max(np.abs(res)) / (knorm_inf * np.max(np.abs(x)) + np.max(np.abs(rhs))))
+    dA.free()
+    ph = np.array(phases).mean(axis=0)
+    flops = N ** 3 / 3.0
+    ms = float(np.median(step_ms))
+    upd_flops = (n - 1) * n * (2 * n - 1) / 6.0 * b ** 3  # sum_{m=1}^{n-1} m^2 b^3
+    nl = int(launches[2])
+    achieved = upd_flops / (ph[2] / 1e3) / 1e12
+    return {
+        "N": N, "b": b, "d": d, "seed": seed, "ell": ell, "sf2": sf2, "sn2": sn2,
+        "ms_per_step": ms, "step_ms": step_ms, "tflops_step": flops / (ms / 1e3) / 1e12,
+        "phase_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])},
+        "factor_ms": float(ph.sum()), "tflops_factor": flops / (ph.sum() / 1e3) / 1e12,
+        "update": {"launches": nl, "flops_per_launch": upd_flops / nl, "avg_launch_ms": float(ph[2]) / nl,
+                   "tflops": achieved},
+        "rel_residual": rel_res, "backward_error_inf": bwd,
+    }
+
+
+def cg_workload(hs, dev, syn, args):
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cg"]
+    n = -(-N // b)
+    X, y = syn.gp_inputs(seed, N, d)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    plen = packed_len(N, b)
+    dX, drhs, dx = dev.upload(X.reshape(-1)), dev.upload(rhs), dev.zeros(n * b)
+    dA = dev.alloc(plen * 8)
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+    timer = hs.Timer(dev)
+    # mat-vec alone
+    dq = dev.zeros(n * b)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dq.p, N, b)
+    reps = 20
+    timer.start()
+    for _ in range(reps):
+        hs.call("hs_symv_packed_f64", dev.h, dA.p, drhs.p, dq.p, N, b)
+    timer.stop()
+    symv_ms = timer.ms() / reps
+    it, df, d0 = C.c_int64(), C.c_double(), C.c_double()
+    timer.start()
+    hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100000, C.byref(it), C.byref(df), C.byref(d0))
+    timer.stop()
+    cg_ms = timer.ms()
+    iters = it.value
+    nsymv = iters + -(-iters // 50)
+    x = dev.download(dx, n * b)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dq.p, N, b)
+    rel_res = float(np.linalg.norm(rhs - dev.download(dq, n * b)) / np.linalg.norm(rhs))
+    dA.free()
+    bytes_symv = plen * 8 + 16 * n * b
+    log("cg workload N=%d b=%d: %d iterations in %.1f ms, symv %.3f ms (%.0f GB/s)" %
+        (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6))
+    return {
+        "N": N, "b": b, "seed": seed, "eps": 1e-10, "iterations": iters, "ms": cg_ms,
+        "symv_count": nsymv, "symv_ms": symv_ms, "symv_gbs": bytes_symv / (symv_ms / 1e3) / 1e9,
+        "cg_gbs": nsymv * bytes_symv / (cg_ms / 1e3) / 1e9, "rel_residual_true": rel_res,
+        "delta_ratio": df.value / d0.value if d0.value else 0.0,
+    }
+
+
+def cpu_baseline(args):
+    """The oracle (C restatement of the reference's CPU path, OpenMP) on a
+    bounded sample of the same workload: the N = 8192 system of the same
+    construction (d = 16, b = 256, l = 2, noise 1e-3)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from hs_testlib import Oracle
+
+    syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
+    o = Oracle()
+    seed, _, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
+    N = args.cpu_n
+    X, y = syn.gp_inputs(seed, N, d)
+    t0 = time.time()
+    A = o.rbf(X, N, d, b, ell, sf2, sn2)
+    L = o.cholesky(A, N, b)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    o.tri_solve(L, rhs, N, b)
+    dt = time.time() - t0
+    return {"value": N ** 3 / 3.0 / dt / 1e12, "unit": "TFLOP/s", "cores": int(o.or_num_threads()), "kind": "port",
+            "sample": "oracle (C/OpenMP restatement of the reference CPU path): assembly + blocked Cholesky + "
+                      "2 triangular solves at N=%d, b=%d, d=%d, %.1f s" % (N, b, d, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=0, help="override N of the Cholesky workload (testing only)")
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=8192)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would "
+              "shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    hs = _load("hs_amd", os.path.join(PKG, "hs_amd.py"))
+    syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
+    with hs.Device(0) as dev:
+        ch = cholesky_workload(hs, dev, syn, args)
+        cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
+    cpu = None if args.no_cpu else cpu_baseline(args)
+    up = ch["update"]
+    out = {
+        "metric": METRIC,
+        "value": ch["tflops_step"],
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ch["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic GP inputs (SplitMix64 seed %d, X ~ U[-1,1]^{N x d})" % ch["seed"],
+        "config": {"workload": "FP64 RBF assembly + blocked Cholesky + 2 triangular solves",
+                   "N": ch["N"], "matrix_bsz": ch["b"], "d": ch["d"], "lengthscale": ch["ell"],
+                   "signal_variance": ch["sf2"], "noise_variance": ch["sn2"], "parallelism": "1 GPU"},
+        "roofline": {"kernel": "trailing_mfma_f64 (fused SYRK+GEMM trailing update)", "bound": "mfma",
+                     "achieved": up["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": up["tflops"] / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": up["flops_per_launch"], "avg_launch_ms": up["avg_launch_ms"],
+                     "launches": up["launches"]},
+        "cpu_baseline": cpu,
+        "cholesky": ch,
+        "cg": cg,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,467 @@
+// Symmetric packed mat-vec that reads every stored block once, and the
+// device-resident CG loop built on it.
+//
+// SYMV (replaces matrixVectorBlock_GPU, MatrixVectorOperations.cpp:95-192,
+// which reads every off-diagonal block twice, once per triangle):
+//   pass 1, one work-group per stored block (I,J), in packed (memory) order:
+//     rowpart[ID] = A_IJ x_J           (b values)
+//     colpart[ID] = A_IJ^T x_I         (b values, I > J only)
+//   pass 2, one thread per row i of block row I, fixed order over J:
+//     y_i = sum_{J<=I} rowpart[ID(I,J)]_i + sum_{J>I} colpart[ID(J,I)]_i
+// Lanes read 16 contiguous bytes each, so one wave load covers 1 KB of a
+// block; row sums are finished by a recursive-halving shuffle reduction,
+// column sums stay in registers until a 4-wave LDS combine.
+// The result is bit-reproducible run to run (no atomics).
+#include <algorithm>
+#include <climits>
+
+#include "hs_internal.h"
+
+int hs_launch_dot(hs_handle_t h, const double* x, const double* y, i64 begin, i64 len, double* part, int maxG,
+                  double* out);
+
+namespace {
+
+constexpr int SYMV_THREADS = 256;
+
+// Recursive-halving reduction of R values per lane across aligned groups of
+// G lanes. On return v[0] of lane l holds the group total of value index
+// (l & (G-1)) >> log2(G/R).
+template <int R, int G>
+__device__ inline double halving_reduce(double (&v)[R]) {
+    const int lane = threadIdx.x & 63;
+    int len = R;
+#pragma unroll
+    for (int m = G / 2; m >= G / R; m >>= 1) {
+        const int half = len >> 1;
+        const bool upper = (lane & m) != 0;
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) {
+            if (i < half) {
+                const double send = upper ? v[i] : v[i + half];
+                const double keep = upper ? v[i + half] : v[i];
+                v[i] = keep + __shfl_xor(send, m, 64);
+            }
+        }
+        len = half;
+    }
+    double s = v[0];
+#pragma unroll
+    for (int m = G / R / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    return s;
+}
+
+__device__ inline void decode_block(i64 id, int n, int* I, int* J) {
+    // column J starts at S(J) = J*n - J(J-1)/2
+    const double nn = 2.0 * n + 1.0;
+    i64 j = (i64)((nn - sqrt(nn * nn - 8.0 * (double)id)) * 0.5);
+    if (j < 0) j = 0;
+    while (j > 0 && j * n - j * (j - 1) / 2 > id) --j;
+    while ((j + 1) * n - (j + 1) * j / 2 <= id) ++j;
+    *J = (int)j;
+    *I = (int)(j + (id - (j * n - j * (j - 1) / 2)));
+}
+
+template <int BS>
+__global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const double* __restrict__ A,
+                                                                      const double* __restrict__ x,
+                                                                      double* __restrict__ rowpart,
+                                                                      double* __restrict__ colpart, int n,
+                                                                      const i64* __restrict__ stop, i64 it) {
+    if (stop && it > *stop) return;
+    constexpr int LPR = BS / 2 < 64 ? BS / 2 : 64;  // lanes per row segment
+    constexpr int RPL = 64 / LPR;                    // rows per wave load
+    constexpr int NL = BS / (2 * LPR);               // double2 loads per row
+    constexpr int RPW = BS / 4;                      // rows per wave
+    constexpr int ITERS = RPW / RPL;                 // wave loads (per segment) per wave
+    constexpr int R = ITERS < 32 ? ITERS : 32;       // row partials reduced per group
+    constexpr int G = LPR;                           // lanes sharing one row
+    __shared__ double colred[4 * RPL][BS];
+
+    const i64 id = blockIdx.x;
+    int I, J;
+    decode_block(id, n, &I, &J);
+    const bool diag = (I == J);
+    const double* blk = A + id * (i64)BS * BS;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sub = lane / LPR;   // which row of a wave load
+    const int sl = lane % LPR;    // lane within the row segment
+    const double* xJ = x + (i64)J * BS;
+    const double* xI = x + (i64)I * BS;
+
+    double xj[2 * NL];
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+        xj[2 * s] = xJ[s * 2 * LPR + 2 * sl];
+        xj[2 * s + 1] = xJ[s * 2 * LPR + 2 * sl + 1];
+    }
+    double cacc[2 * NL];
+#pragma unroll
+    for (int s = 0; s < 2 * NL; ++s) cacc[s] = 0.0;
+
+    const int rowBase = wave * RPW;
+#pragma unroll
+    for (int g0 = 0; g0 < ITERS; g0 += R) {
+        double p[R];
+#pragma unroll
+        for (int it2 = 0; it2 < R; ++it2) {
+            const int row = rowBase + (g0 + it2) * RPL + sub;
+            const double* rp = blk + (i64)row * BS + 2 * sl;
+            double acc = 0.0;
+            const double xi = xI[row];
+#pragma unroll
+            for (int s = 0; s < NL; ++s) {
+                const double2 a = *reinterpret_cast<const double2*>(rp + s * 2 * LPR);
+                acc += a.x * xj[2 * s] + a.y * xj[2 * s + 1];
+                cacc[2 * s] += a.x * xi;
+                cacc[2 * s + 1] += a.y * xi;
+            }
+            p[it2] = acc;
+        }
+        const double tot = halving_reduce<R, G>(p);
+        const int idx = sl >> (__builtin_ctz(G / R));
+        if ((sl & (G / R - 1)) == 0) {
+            const int row = rowBase + (g0 + idx) * RPL + sub;
+            rowpart[id * BS + row] = tot;
+        }
+    }
+    if (diag) return;  // diagonal blocks hold both triangles: row form only
+#pragma unroll
+    for (int s = 0; s < NL; ++s) {
+        colred[wave * RPL + sub][s * 2 * LPR + 2 * sl] = cacc[2 * s];
+        colred[wave * RPL + sub][s * 2 * LPR + 2 * sl + 1] = cacc[2 * s + 1];
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < BS; c += SYMV_THREADS) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4 * RPL; ++w) s += colred[w][c];
+        colpart[id * BS + c] = s;
+    }
+}
+
+// pass 2 variants: 0 = y = sum; 1 = y = sum and dqpart[wg] = sum_i y_i d_i;
+// 2 = y = rhs - sum and part[wg] = sum_i y_i^2
+template <int MODE>
+__global__ __launch_bounds__(SYMV_THREADS) void symv_finish_kernel(const double* __restrict__ rowpart,
+                                                                    const double* __restrict__ colpart, int n, int b,
+                                                                    double* __restrict__ y,
+                                                                    const double* __restrict__ other,
+                                                                    double* __restrict__ part,
+                                                                    const i64* __restrict__ stop, i64 it) {
+    if (stop && it > *stop) return;
+    __shared__ double red[SYMV_THREADS];
+    const i64 i = (i64)blockIdx.x * SYMV_THREADS + threadIdx.x;
+    const i64 rows = (i64)n * b;
+    double v = 0.0;
+    if (i < rows) {
+        const int I = (int)(i / b), ii = (int)(i % b);
+        double s = 0.0;
+        for (int J = 0; J <= I; ++J) s += rowpart[hs_block_id(I, J, n) * b + ii];
+        const i64 base = hs_block_id(I, I, n);
+        for (int J = I + 1; J < n; ++J) s += colpart[(base + (J - I)) * b + ii];
+        if (MODE == 2) s = other[i] - s;
+        y[i] = s;
+        if (MODE == 1) v = s * other[i];
+        if (MODE == 2) v = s * s;
+    }
+    if (MODE == 0) return;
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = SYMV_THREADS / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// every work-group of the vector kernels sums the G partials in the same
+// fixed tree order, so all of them see bit-identical scalars
+__device__ inline double wg_sum_partials(const double* __restrict__ part, int G, double* red) {
+    double a = 0.0;
+    for (int g = threadIdx.x; g < G; g += SYMV_THREADS) a += part[g];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = SYMV_THREADS / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+struct CGState {
+    double delta[2];  // delta_new of even / odd iteration parity
+    double delta0;
+    double eps2;
+    double alpha;
+    double pad;
+    i64 stop;  // last executed iteration; LLONG_MAX while running
+};
+
+__global__ __launch_bounds__(SYMV_THREADS) void cg_init_kernel(const double* __restrict__ rhs, double* __restrict__ x,
+                                                                double* __restrict__ r, double* __restrict__ d, i64 rows,
+                                                                double* __restrict__ part) {
+    __shared__ double red[SYMV_THREADS];
+    const i64 i = (i64)blockIdx.x * SYMV_THREADS + threadIdx.x;
+    double v = 0.0;
+    if (i < rows) {
+        // x0 = 0 so r = b - A x0 = b exactly (CG.cpp:277-306)
+        const double bi = rhs[i];
+        x[i] = 0.0;
+        r[i] = bi;
+        d[i] = bi;
+        v = bi * bi;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = SYMV_THREADS / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(SYMV_THREADS) void cg_init_finish_kernel(const double* __restrict__ part, int G,
+                                                                       CGState* st, double eps2) {
+    __shared__ double red[SYMV_THREADS];
+    const double s = wg_sum_partials(part, G, red);
+    if (threadIdx.x == 0) {
+        st->delta[0] = s;
+        st->delta[1] = s;
+        st->delta0 = s;
+        st->eps2 = eps2;
+        st->alpha = 0.0;
+        st->stop = (s > eps2 * s) ? LLONG_MAX : -1;
+    }
+}
+
+// alpha = delta / d.q ; x += alpha d ; (it % 50 != 0) r -= alpha q, part = r.r
+__global__ __launch_bounds__(SYMV_THREADS) void cg_xr_kernel(CGState* st, const double* __restrict__ dqpart, int G,
+                                                              const double* __restrict__ d,
+                                                              const double* __restrict__ q, double* __restrict__ x,
+                                                              double* __restrict__ r, i64 rows,
+                                                              double* __restrict__ rrpart, i64 it, int updateR) {
+    if (it > st->stop) return;
+    __shared__ double red[SYMV_THREADS];
+    const double dq = wg_sum_partials(dqpart, G, red);
+    const double alpha = st->delta[it & 1] / dq;
+    const i64 i = (i64)blockIdx.x * SYMV_THREADS + threadIdx.x;
+    double v = 0.0;
+    if (i < rows) {
+        x[i] = x[i] + d[i] * alpha;
+        if (updateR) {
+            const double ri = r[i] + q[i] * (-1.0 * alpha);
+            r[i] = ri;
+            v = ri * ri;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+    if (!updateR) return;
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = SYMV_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rrpart[blockIdx.x] = red[0];
+}
+
+// delta_new = r.r ; beta = delta_new / delta_old ; d = r + beta d ; stop test
+__global__ __launch_bounds__(SYMV_THREADS) void cg_d_kernel(CGState* st, const double* __restrict__ rrpart, int G,
+                                                             const double* __restrict__ r, double* __restrict__ d,
+                                                             i64 rows, i64 it) {
+    if (it > st->stop) return;
+    __shared__ double red[SYMV_THREADS];
+    const double dn = wg_sum_partials(rrpart, G, red);
+    const double beta = dn / st->delta[it & 1];
+    const i64 i = (i64)blockIdx.x * SYMV_THREADS + threadIdx.x;
+    if (i < rows) d[i] = r[i] + d[i] * beta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->delta[(it + 1) & 1] = dn;
+        if (!(dn > st->eps2 * st->delta0)) st->stop = it;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static bool symv_fast_ok(int b) { return b == 64 || b == 128 || b == 256; }
+
+static int launch_partials(hs_handle_t h, const double* A, const double* x, double* rowpart, double* colpart, int n,
+                           int b, const i64* stop, i64 it) {
+    const i64 blocks = (i64)n * (n + 1) / 2;
+    if (blocks > 0x7fffffffLL) {
+        hs_set_error("SYMV: too many blocks");
+        return HS_ERR_UNSUPPORTED;
+    }
+    dim3 grid((unsigned)blocks), blk(SYMV_THREADS);
+    switch (b) {
+        case 64: symv_partials_kernel<64><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        case 128: symv_partials_kernel<128><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        case 256: symv_partials_kernel<256><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        default: hs_set_error("SYMV: unsupported block size"); return HS_ERR_UNSUPPORTED;
+    }
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+// workspace layout for SYMV / CG (doubles)
+struct SymvWork {
+    double* rowpart;
+    double* colpart;
+    double* part1;
+    double* part2;
+    double* vec;   // 3 vectors: d, q, r for CG
+    CGState* st;
+};
+
+static int symv_workspace(hs_handle_t h, int n, int b, bool withVectors, SymvWork* w) {
+    const i64 nb = (i64)n * (n + 1) / 2;
+    const i64 rows = (i64)n * b;
+    const i64 G = (rows + SYMV_THREADS - 1) / SYMV_THREADS;
+    const size_t bytes = sizeof(double) * (size_t)(2 * nb * b + 2 * (G + 64) + (withVectors ? 3 * rows : 0)) +
+                         sizeof(CGState) + 256;
+    int rc = hs_ensure_work(h, bytes);
+    if (rc) return rc;
+    double* p = (double*)h->work;
+    w->rowpart = p;
+    p += nb * b;
+    w->colpart = p;
+    p += nb * b;
+    w->part1 = p;
+    p += G + 64;
+    w->part2 = p;
+    p += G + 64;
+    w->vec = p;
+    if (withVectors) p += 3 * rows;
+    w->st = (CGState*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
+    return HS_OK;
+}
+
+extern "C" int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (!x) return -3;
+    if (!y) return -4;
+    if (N <= 0) return -5;
+    if (b <= 0) return -6;
+    const int n = (int)((N + b - 1) / b);
+    if (!symv_fast_ok(b)) return hs_symv_block_f64(h, A, x, y, b, 0, 0, n, n, n, 1);
+    SymvWork w;
+    int rc = symv_workspace(h, n, b, false, &w);
+    if (rc) return rc;
+    if ((rc = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, nullptr, 0))) return rc;
+    const i64 rows = (i64)n * b;
+    const i64 G = (rows + SYMV_THREADS - 1) / SYMV_THREADS;
+    symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(w.rowpart, w.colpart, n, b, y,
+                                                                                   nullptr, nullptr, nullptr, 0);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+// CG at share 1.0 (CG.cpp:38-163, 277-528). The host enqueues iterations in
+// chunks; each iteration's kernels check the device stop index first, so
+// iterations enqueued past convergence cost only their launch. The stop
+// index is read back once per chunk through pinned memory.
+extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                         int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (!rhs) return -3;
+    if (!x) return -4;
+    if (N <= 0) return -5;
+    if (!symv_fast_ok(b)) {
+        hs_set_error("hs_cg_f64: block size must be 64, 128 or 256");
+        return -6;
+    }
+    if (!(eps >= 0.0)) return -7;
+    if (imax < 0) return -8;
+    const int n = (int)((N + b - 1) / b);
+    const i64 rows = (i64)n * b;
+    const int G = (int)((rows + SYMV_THREADS - 1) / SYMV_THREADS);
+    SymvWork w;
+    int rc = symv_workspace(h, n, b, true, &w);
+    if (rc) return rc;
+    double* d = w.vec;
+    double* q = w.vec + rows;
+    double* r = w.vec + 2 * rows;
+    CGState* st = w.st;
+    const i64* stop = &st->stop;
+    const dim3 vg((unsigned)G), vb(SYMV_THREADS);
+
+    cg_init_kernel<<<vg, vb, 0, h->stream>>>(rhs, x, r, d, rows, w.part2);
+    HS_LAUNCH_CHECK();
+    cg_init_finish_kernel<<<1, vb, 0, h->stream>>>(w.part2, G, st, eps * eps);
+    HS_LAUNCH_CHECK();
+
+    const int CHUNK = 8;
+    volatile i64* pinnedStop = (volatile i64*)h->pinned;
+    i64 enq = 0;
+    i64 stopHost = LLONG_MAX;
+    hipEvent_t evs[2];
+    HS_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+    HS_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    int pending = -1;  // slot of the chunk whose stop copy is in flight
+    auto enqueue_chunk = [&]() -> int {
+        const i64 end = std::min<i64>(enq + CHUNK, imax);
+        for (; enq < end; ++enq) {
+            const i64 it = enq;
+            int e;
+            if ((e = launch_partials(h, A, d, w.rowpart, w.colpart, n, b, stop, it))) return e;
+            symv_finish_kernel<1><<<vg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, q, d, w.part1, stop, it);
+            HS_LAUNCH_CHECK();
+            const int updateR = (it % 50) != 0;
+            cg_xr_kernel<<<vg, vb, 0, h->stream>>>(st, w.part1, G, d, q, x, r, rows, w.part2, it, updateR);
+            HS_LAUNCH_CHECK();
+            if (!updateR) {
+                // true residual every 50 iterations (CG.cpp:95-101)
+                if ((e = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, stop, it))) return e;
+                symv_finish_kernel<2><<<vg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, r, rhs, w.part2, stop,
+                                                                it);
+                HS_LAUNCH_CHECK();
+            }
+            cg_d_kernel<<<vg, vb, 0, h->stream>>>(st, w.part2, G, r, d, rows, it);
+            HS_LAUNCH_CHECK();
+        }
+        return HS_OK;
+    };
+    int slot = 0;
+    while (true) {
+        if (enq < imax) {
+            if ((rc = enqueue_chunk())) break;
+            HS_HIP(hipMemcpyAsync((void*)(pinnedStop + slot), stop, sizeof(i64), hipMemcpyDeviceToHost, h->stream));
+            HS_HIP(hipEventRecord(evs[slot], h->stream));
+        }
+        if (pending >= 0) {
+            HS_HIP(hipEventSynchronize(evs[pending]));
+            stopHost = pinnedStop[pending];
+            if (stopHost != LLONG_MAX) break;
+        }
+        if (enq >= imax) {
+            // drain the last chunk's copy
+            if (pending != slot) {
+                HS_HIP(hipEventSynchronize(evs[slot]));
+                stopHost = pinnedStop[slot];
+            }
+            break;
+        }
+        pending = slot;
+        slot ^= 1;
+    }
+    hipEventDestroy(evs[0]);
+    hipEventDestroy(evs[1]);
+    if (rc) return rc;
+    HS_HIP(hipStreamSynchronize(h->stream));
+    CGState hs;
+    HS_HIP(hipMemcpy(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost));
+    const i64 iters = (hs.stop == LLONG_MAX) ? enq : hs.stop + 1;
+    if (iterations) *iterations = iters;
+    if (delta_final) *delta_final = hs.delta[iters & 1];
+    if (delta_zero) *delta_zero = hs.delta0;
+    return HS_OK;
+}

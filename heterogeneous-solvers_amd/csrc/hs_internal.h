@@ -1,0 +1,65 @@
+// Internal declarations shared by the libhs_amd.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "hs_abi.h"
+
+typedef int64_t i64;
+
+// Scratch sizes are fixed per handle once grown; kernels never allocate.
+struct hs_handle_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int* info_dev = nullptr;   // sticky first bad pivot (1-based global row), 0 = none
+    // grow-only device workspace for partial sums / scalars
+    void* work = nullptr;
+    size_t work_bytes = 0;
+    // pinned host word used to poll device-side loop state without a full copy
+    void* pinned = nullptr;
+    int cu_count = 256;
+};
+
+struct hs_event_s {
+    hipEvent_t ev = nullptr;
+};
+
+void hs_set_error(const std::string& msg);
+int hs_hip_fail(hipError_t e, const char* what);
+// Ensures the handle workspace holds at least `bytes`; called only from
+// non-hot set-up points (first call of an entry on a size).
+int hs_ensure_work(hs_handle_t h, size_t bytes);
+
+#define HS_HIP(call)                                                \
+    do {                                                            \
+        hipError_t _e = (call);                                     \
+        if (_e != hipSuccess) return hs_hip_fail(_e, #call);        \
+    } while (0)
+
+#define HS_LAUNCH_CHECK() HS_HIP(hipGetLastError())
+
+// packed block ID of (I,J), I >= J
+__host__ __device__ inline i64 hs_block_id(i64 I, i64 J, i64 n) { return J * n - J * (J - 1) / 2 + (I - J); }
+
+// Row-by-row decode of a linear index t over a lower triangle with diagonal:
+// t = r(r+1)/2 + c, 0 <= c <= r.
+__host__ __device__ inline void hs_tri_decode(i64 t, int* r, int* c) {
+    i64 rr = (i64)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (rr * (rr + 1) / 2 > t) --rr;
+    while ((rr + 1) * (rr + 2) / 2 <= t) ++rr;
+    *r = (int)rr;
+    *c = (int)(t - rr * (rr + 1) / 2);
+}
+
+// kernels implemented in other translation units, used by the solver drivers
+int hs_launch_potrf(hs_handle_t h, double* A, i64 N, int b, i64 blockID, int blockRow);
+int hs_launch_trsm(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
+// trailing update over rows [rowStart, rowStart+rowCount) of step k;
+// mode 1 = diagonal blocks only (SYRK), 2 = off-diagonal only (GEMM), 3 = both
+int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode);
+int hs_launch_trsv_block(hs_handle_t h, const double* A, double* x, i64 N, int b, int blockRow, i64 blockID, int transposed);
+int hs_launch_col_update(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
+                         i64 blockID, int n, int transposed);

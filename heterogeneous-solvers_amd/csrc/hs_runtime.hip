@@ -1,0 +1,203 @@
+// Runtime part of the C ABI: handles, memory, copies, events, errors.
+// This is the stand-in for the reference's sycl::queue + USM calls
+// (main.cpp:189-192, Cholesky.cpp:24-32, CG.cpp:165-234).
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hs_internal.h"
+
+static thread_local std::string g_last_error;
+
+void hs_set_error(const std::string& msg) { g_last_error = msg; }
+
+int hs_hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return HS_ERR_HIP;
+}
+
+extern "C" const char* hs_last_error_string(void) { return g_last_error.c_str(); }
+
+extern "C" int hs_device_count(int* count) {
+    if (!count) return -1;
+    HS_HIP(hipGetDeviceCount(count));
+    return HS_OK;
+}
+
+extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
+    if (!handle) return -1;
+    int count = 0;
+    HS_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) {
+        hs_set_error("hs_handle_create: no such device");
+        return -2;
+    }
+    HS_HIP(hipSetDevice(device));
+    hs_handle_s* h = new hs_handle_s();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hs_hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cu_count = prop.multiProcessorCount;
+    e = hipMalloc((void**)&h->info_dev, sizeof(int) * 2);
+    if (e == hipSuccess) e = hipMemset(h->info_dev, 0, sizeof(int) * 2);
+    if (e == hipSuccess) e = hipHostMalloc(&h->pinned, 4096, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipStreamDestroy(h->stream);
+        delete h;
+        return hs_hip_fail(e, "hs_handle_create allocations");
+    }
+    *handle = h;
+    return HS_OK;
+}
+
+extern "C" int hs_handle_destroy(hs_handle_t h) {
+    if (!h) return -1;
+    hipSetDevice(h->device);
+    hipStreamSynchronize(h->stream);
+    if (h->work) hipFree(h->work);
+    if (h->info_dev) hipFree(h->info_dev);
+    if (h->pinned) hipHostFree(h->pinned);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return HS_OK;
+}
+
+extern "C" int hs_sync(hs_handle_t h) {
+    if (!h) return -1;
+    HS_HIP(hipStreamSynchronize(h->stream));
+    return HS_OK;
+}
+
+int hs_ensure_work(hs_handle_t h, size_t bytes) {
+    if (bytes <= h->work_bytes) return HS_OK;
+    // growth happens between calls, never inside a launch sequence
+    HS_HIP(hipStreamSynchronize(h->stream));
+    if (h->work) HS_HIP(hipFree(h->work));
+    h->work = nullptr;
+    h->work_bytes = 0;
+    size_t sz = bytes + (bytes >> 3) + 4096;
+    hipError_t e = hipMalloc(&h->work, sz);
+    if (e != hipSuccess) {
+        h->work = nullptr;
+        hs_set_error("hs workspace allocation failed");
+        return HS_ERR_NOMEM;
+    }
+    h->work_bytes = sz;
+    return HS_OK;
+}
+
+extern "C" int hs_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return -1;
+    hipError_t e = hipMalloc(ptr, bytes == 0 ? 1 : bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        hs_hip_fail(e, "hipMalloc");
+        return HS_ERR_NOMEM;
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_free(void* ptr) {
+    if (ptr) HS_HIP(hipFree(ptr));
+    return HS_OK;
+}
+
+extern "C" int hs_memcpy_htod(hs_handle_t h, void* dst, const void* src, size_t bytes) {
+    if (!h) return -1;
+    HS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HS_HIP(hipStreamSynchronize(h->stream));
+    return HS_OK;
+}
+
+extern "C" int hs_memcpy_dtoh(hs_handle_t h, void* dst, const void* src, size_t bytes) {
+    if (!h) return -1;
+    HS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    HS_HIP(hipStreamSynchronize(h->stream));
+    return HS_OK;
+}
+
+extern "C" int hs_memcpy_dtod(hs_handle_t h, void* dst, const void* src, size_t bytes) {
+    if (!h) return -1;
+    HS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return HS_OK;
+}
+
+__global__ void hs_zero_kernel(double* p, i64 n64, unsigned char* tail, int ntail) {
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 stride = (i64)gridDim.x * blockDim.x;
+    for (; i < n64; i += stride) p[i] = 0.0;
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+// A fill kernel rather than hipMemsetAsync, so callers may capture it in a graph.
+extern "C" int hs_memset_zero(hs_handle_t h, void* dst, size_t bytes) {
+    if (!h) return -1;
+    if (bytes == 0) return HS_OK;
+    if (((uintptr_t)dst & 7) != 0) {
+        hs_set_error("hs_memset_zero: destination must be 8-byte aligned");
+        return -2;
+    }
+    i64 n64 = (i64)(bytes / 8);
+    int ntail = (int)(bytes % 8);
+    i64 blocks = (n64 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hs_zero_kernel<<<dim3((unsigned)blocks), dim3(256), 0, h->stream>>>((double*)dst, n64,
+                                                                         (unsigned char*)dst + n64 * 8, ntail);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+extern "C" int hs_event_create(hs_event_t* ev) {
+    if (!ev) return -1;
+    hs_event_s* e = new hs_event_s();
+    hipError_t r = hipEventCreate(&e->ev);
+    if (r != hipSuccess) {
+        delete e;
+        return hs_hip_fail(r, "hipEventCreate");
+    }
+    *ev = e;
+    return HS_OK;
+}
+
+extern "C" int hs_event_destroy(hs_event_t ev) {
+    if (!ev) return -1;
+    hipEventDestroy(ev->ev);
+    delete ev;
+    return HS_OK;
+}
+
+extern "C" int hs_event_record(hs_handle_t h, hs_event_t ev) {
+    if (!h) return -1;
+    if (!ev) return -2;
+    HS_HIP(hipEventRecord(ev->ev, h->stream));
+    return HS_OK;
+}
+
+extern "C" int hs_event_elapsed_ms(hs_event_t start, hs_event_t stop, float* ms) {
+    if (!start) return -1;
+    if (!stop) return -2;
+    if (!ms) return -3;
+    HS_HIP(hipEventSynchronize(stop->ev));
+    HS_HIP(hipEventElapsedTime(ms, start->ev, stop->ev));
+    return HS_OK;
+}
+
+extern "C" int hs_get_info(hs_handle_t h, int64_t* info) {
+    if (!h) return -1;
+    if (!info) return -2;
+    int v = 0;
+    HS_HIP(hipMemcpyAsync(&v, h->info_dev, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HS_HIP(hipStreamSynchronize(h->stream));
+    *info = v;
+    return HS_OK;
+}
+
+extern "C" int hs_reset_info(hs_handle_t h) {
+    if (!h) return -1;
+    return hs_memset_zero(h, h->info_dev, 8);
+}

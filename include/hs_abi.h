@@ -1,0 +1,174 @@
+/*
+ * hs_abi.h -- C ABI of the MI355X (gfx950) blocked SPD solver library
+ * libhs_amd.so.
+ *
+ * Plain C types only: device pointers are ordinary pointers obtained from
+ * hs_malloc (or any hipMalloc'd memory), sizes are int64_t, every entry
+ * returns an int status.
+ *
+ * Storage contract (same as the reference's SymmetricMatrix,
+ * src/dataStructures/SymmetricMatrix.hpp:9-35): the lower block triangle of
+ * an N x N SPD matrix, split into b x b blocks, n = ceil(N/b) block rows.
+ * Block (I,J), I >= J, has ID  J*n - J*(J-1)/2 + (I-J); it is stored row
+ * major at element offset ID*b*b. Diagonal blocks hold both triangles.
+ * Rows/columns >= N are zero padding. Vectors have n*b entries.
+ *
+ * Status codes: 0 = ok; -i = argument i invalid (LAPACK style);
+ * > 0 from a factorisation = 1-based global index of the first
+ * non-positive pivot (the reference produces NaN silently there,
+ * MatrixOperations.cpp:24); HS_ERR_* below for runtime failures, with
+ * hs_last_error_string() describing the last one on the calling thread.
+ *
+ * Threading: a handle owns one HIP stream and its scratch workspace. Calls on
+ * one handle are asynchronous and ordered by that stream (this replaces the
+ * reference's host wait() after every kernel); hs_sync() is the barrier.
+ * A handle is not re-entrant: use one handle per host thread.
+ */
+#ifndef HS_ABI_H
+#define HS_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_OK 0
+#define HS_ERR_HIP (-1000)
+#define HS_ERR_NOMEM (-1001)
+#define HS_ERR_UNSUPPORTED (-1002)
+#define HS_ERR_NOT_CONVERGED (-1003)
+
+typedef struct hs_handle_s* hs_handle_t;
+typedef struct hs_event_s* hs_event_t;
+
+/* ---- runtime ------------------------------------------------------------
+ * Stand-in for the reference's sycl::queue (main.cpp:189-192): a device and
+ * an in-order stream. */
+int hs_handle_create(hs_handle_t* handle, int device);
+int hs_handle_destroy(hs_handle_t handle);
+int hs_sync(hs_handle_t handle);
+const char* hs_last_error_string(void);
+int hs_device_count(int* count);
+/* Replaces sycl::malloc_device / free / queue.memcpy (Cholesky.cpp:24-32, CG.cpp:204-229). */
+int hs_malloc(void** ptr, size_t bytes);
+int hs_free(void* ptr);
+int hs_memcpy_htod(hs_handle_t handle, void* dst, const void* src, size_t bytes);
+int hs_memcpy_dtoh(hs_handle_t handle, void* dst, const void* src, size_t bytes);
+int hs_memcpy_dtod(hs_handle_t handle, void* dst, const void* src, size_t bytes);
+int hs_memset_zero(hs_handle_t handle, void* dst, size_t bytes);
+/* Timing events on the handle's stream (the reference reads SYCL event
+ * profiling, CG.cpp:386-397, Cholesky.cpp:292-353). */
+int hs_event_create(hs_event_t* ev);
+int hs_event_destroy(hs_event_t ev);
+int hs_event_record(hs_handle_t handle, hs_event_t ev);
+int hs_event_elapsed_ms(hs_event_t start, hs_event_t stop, float* ms);
+
+/* ---- kernel ops: one entry per reference L3 kernel ----------------------
+ * Arguments keep the reference's meaning and order after the handle. */
+
+/* POTRF of diagonal block `blockID` (block row `blockRow`), in place, upper
+ * triangle zeroed, rows >= N untouched.
+ * Replaces MatrixOperations::cholesky_optimizedGPU (MatrixOperations.cpp:124-191)
+ * and ::cholesky (:7-62). A non-positive pivot is recorded; read it with
+ * hs_get_info. */
+int hs_potrf_diag_f64(hs_handle_t h, double* A, int64_t N, int b, int blockID, int blockRow);
+/* Sticky first-bad-pivot (1-based global row, 0 = none) since the last reset. */
+int hs_get_info(hs_handle_t h, int64_t* info);
+int hs_reset_info(hs_handle_t h);
+
+/* B <- B L^{-T} for blockCount blocks of block column blockRow starting at
+ * block row blockStart. Replaces MatrixMatrixOperations::triangularSolve_optimizedGPU
+ * (MatrixMatrixOperations.cpp:50-107). */
+int hs_trsm_panel_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart, int blockCount);
+
+/* D <- D - C C^T (lower triangle) on diagonal blocks blockStart..+blockCount-1.
+ * Replaces symmetricMatrixMatrixDiagonal_optimizedGPU (MatrixMatrixOperations.cpp:212-304). */
+int hs_syrk_diag_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart, int blockCount, int blockCountXY);
+
+/* A_IJ <- A_IJ - L_Ik L_Jk^T, k = blockRow, I in [blockStart, blockStart+blockCount),
+ * blockRow < J < I. Replaces matrixMatrixStep_optimizedGPU3 (MatrixMatrixOperations.cpp:645-778). */
+int hs_gemm_trailing_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart, int blockCount, int blockCountXY);
+
+/* Fused trailing update of right-looking step k: SYRK on every diagonal block
+ * and GEMM on every off-diagonal block of the trailing triangle (blocks
+ * (I,J), k < J <= I < n), FP64 MFMA. */
+int hs_syrk_gemm_trailing_f64(hs_handle_t h, double* A, int b, int k, int blockCountXY);
+
+/* result = A x on a sub-range of block rows/columns (reset=0 accumulates).
+ * Replaces MatrixVectorOperations::matrixVectorBlock{,_GPU}
+ * (MatrixVectorOperations.cpp:8-192). */
+int hs_symv_block_f64(hs_handle_t h, const double* A, const double* x, double* result, int b, int blockStart_i,
+                      int blockStart_j, int blockCount_i, int blockCount_j, int blockCountXY, int reset);
+
+/* y = A x over the whole packed matrix, reading every stored block once and
+ * applying it to both triangles; deterministic (fixed-order) reduction.
+ * Replaces matrixVectorBlock_GPU at share 1.0 (CG.cpp:364-369). */
+int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b);
+
+/* x_J <- L_JJ^{-1} x_J (transposed: L_JJ^{-T} x_J), entries >= N untouched.
+ * Replaces MatrixVectorOperations::triangularSolveBlockVector (:282-335). */
+int hs_trsv_block_f64(hs_handle_t h, const double* A, double* x, int64_t N, int b, int blockRow, int blockID, int transposed);
+
+/* x_I -= L_{I,blockRow} x_blockRow (transposed: L_{blockRow,I}^T x_blockRow)
+ * for I in [blockStart, blockStart+blockCount).
+ * Replaces MatrixVectorOperations::matrixVectorColumnUpdate (:337-377). */
+int hs_col_update_f64(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
+                      int blockID, int blockCountXY, int transposed);
+
+/* Vector block ops on entries [blockStart*b, (blockStart+blockCount)*b).
+ * Replace VectorOperations::scaleVectorBlock, addVectorBlock, subVectorBlock,
+ * scaleAndAddVectorBlock (VectorOperations.cpp:6-82). */
+int hs_vec_scale_f64(hs_handle_t h, const double* x, double alpha, double* result, int b, int blockStart, int blockCount);
+int hs_vec_add_f64(hs_handle_t h, const double* x, const double* y, double* result, int b, int blockStart, int blockCount);
+int hs_vec_sub_f64(hs_handle_t h, const double* x, const double* y, double* result, int b, int blockStart, int blockCount);
+int hs_vec_scale_add_f64(hs_handle_t h, const double* x, double alpha, const double* y, double* result, int b,
+                         int blockStart, int blockCount);
+/* x.y over the block range; fixed-order two-pass reduction; result written
+ * to device memory *result_dev (stays on the device, as the CG loop needs).
+ * Replaces scalarProduct + sumFinalScalarProduct (VectorOperations.cpp:84-185). */
+int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount, double* result_dev);
+
+/* ---- whole-solver paths -------------------------------------------------- */
+
+/* Right-looking blocked Cholesky A -> L in place on the device, stream
+ * ordered, no host waits. Returns 0 or the first bad pivot (>0).
+ * Replaces Cholesky::solve_heterogeneous at share 1.0 (Cholesky.cpp:413-493). */
+int hs_cholesky_f64(hs_handle_t h, double* A, int64_t N, int b);
+
+/* hs_cholesky_f64 with HIP events around every launch: phase_ms[0..2] get
+ * the summed durations of the POTRF, TRSM and trailing-update launches and
+ * launches[0..2] their counts. Replaces the per-step kernel timing the
+ * reference collects into MetricsTracker (Cholesky.cpp:261-359). */
+int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* phase_ms, int64_t* launches);
+
+/* Forward L z = x then backward L^T y = z, in place on x (device).
+ * Replaces TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105). */
+int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b);
+
+/* Conjugate gradients with x0 = 0, true residual at iteration % 50 == 0,
+ * stop when delta_new <= eps^2 delta_0 or after imax iterations; the loop
+ * stays on the device. Outputs may be NULL.
+ * Replaces CG::solveHeterogeneous at share 1.0 (CG.cpp:38-163). */
+int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+              int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero);
+
+/* RBF kernel matrix into the packed layout on the device:
+ * K_ij = sf2 * exp(-||x_i - x_j||^2 / (2 ell^2)) + sn2 [i == j],
+ * X row-major N x d on the device. Padding entries are written as zero.
+ * Replaces MatrixGenerator::generateSPDMatrix's per-block CPU kernel
+ * (MatrixGenerator.cpp:107-159) with one device launch. */
+int hs_rbf_assemble_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, double ell, double sf2, double sn2,
+                        double* A);
+
+/* Reference compatibility features: row i's feature vector is the 8-lag
+ * window (t_{i-7}, ..., t_i) of the device series t (zeros before t_0),
+ * sf2 = 1, ell = 1, sn2 = 0.01, as MatrixGenerator.cpp:77-99, 125-153. */
+int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* A);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HS_ABI_H */

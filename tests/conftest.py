@@ -1,0 +1,36 @@
+import os
+import sys
+
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+from hs_testlib import Oracle, load_hs  # noqa: E402
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and runs through libhs_amd.so")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def hs():
+    return load_hs()
+
+
+@pytest.fixture(scope="session")
+def dev(hs):
+    import ctypes as C
+
+    lib = hs.load()
+    cnt = C.c_int(0)
+    rc = lib.hs_device_count(C.byref(cnt))
+    if rc != 0 or cnt.value == 0:
+        pytest.skip("no GPU visible to the HIP runtime")
+    d = hs.Device(0)
+    yield d
+    d.close()

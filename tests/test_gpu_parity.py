@@ -1,0 +1,169 @@
+"""GPU path against the oracle at the block sizes the solvers run with
+(64 / 128 / 256, MFMA and streaming kernels), on seeded RBF systems small
+enough for the oracle to finish in seconds."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hs_testlib import dense_to_packed, dptr, packed_len, packed_to_dense, synth_gp_numpy
+
+pytestmark = pytest.mark.gpu
+
+
+def rbf_system(oracle, N, d, b, ell=1.0, sf2=1.0, sn2=1e-2, seed=1):
+    X, y = synth_gp_numpy(seed, N, d)
+    A = oracle.rbf(X, N, d, b, ell, sf2, sn2)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    return X, A, rhs
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("N,b", [(1000, 64), (2048, 128), (2560, 256), (300, 20), (4096, 128)])
+def test_symv_packed_matches_oracle(hs, dev, oracle, N, b):
+    say("symv_packed N=%d b=%d" % (N, b))
+    _, A, _ = rbf_system(oracle, N, 8, b)
+    n = -(-N // b)
+    x = np.zeros(n * b)
+    x[:N] = np.random.default_rng(N).standard_normal(N)
+    ref = oracle.symv(A, x, b, n)
+    dA, dx, dy = dev.upload(A), dev.upload(x), dev.zeros(n * b)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dy.p, N, b)
+    y = dev.download(dy, n * b)
+    scale = np.abs(A).max() * np.abs(x).sum()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * scale
+    assert np.all(y[N:] == 0.0)
+    # bit-reproducible run to run
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dy.p, N, b)
+    assert np.array_equal(dev.download(dy, n * b), y)
+
+
+@pytest.mark.parametrize("b,k,n", [(64, 0, 6), (128, 0, 5), (256, 0, 4), (128, 2, 6), (64, 3, 7)])
+def test_trailing_update_mfma_matches_oracle(hs, dev, oracle, b, k, n):
+    say("trailing mfma b=%d k=%d n=%d" % (b, k, n))
+    N = n * b
+    rng = np.random.default_rng(b + k)
+    A = rng.standard_normal(packed_len(N, b))
+    ref = A.copy()
+    bid = n * k - k * (k - 1) // 2
+    oracle.or_syrk(dptr(ref), b, bid, k, k + 1, n - k - 1, n)
+    if n - k - 2 > 0:
+        oracle.or_gemm(dptr(ref), b, bid, k, k + 2, n - k - 2, n)
+    dA = dev.upload(A)
+    hs.call("hs_syrk_gemm_trailing_f64", dev.h, dA.p, b, k, n)
+    got = dev.download(dA, len(A))
+    assert np.max(np.abs(got - ref)) <= 1e-12 * b
+    # separate SYRK / GEMM entries (reference argument conventions)
+    dB = dev.upload(A)
+    hs.call("hs_syrk_diag_f64", dev.h, dB.p, b, bid, k, k + 1, n - k - 1, n)
+    if n - k - 2 > 0:
+        hs.call("hs_gemm_trailing_f64", dev.h, dB.p, b, bid, k, k + 2, n - k - 2, n)
+    assert np.max(np.abs(dev.download(dB, len(A)) - ref)) <= 1e-12 * b
+
+
+@pytest.mark.parametrize("b", [64, 128, 256])
+def test_potrf_and_trsm_match_oracle(hs, dev, oracle, b):
+    say("potrf/trsm b=%d" % b)
+    N = 3 * b
+    _, A, _ = rbf_system(oracle, N, 8, b, sn2=1e-1, seed=b)
+    ref = A.copy()
+    oracle.or_potrf(dptr(ref), N, b, 0, 0)
+    oracle.or_trsm(dptr(ref), b, 0, 0, 1, 2)
+    dA = dev.upload(A)
+    hs.call("hs_reset_info", dev.h)
+    hs.call("hs_potrf_diag_f64", dev.h, dA.p, N, b, 0, 0)
+    hs.call("hs_trsm_panel_f64", dev.h, dA.p, b, 0, 0, 1, 2)
+    got = dev.download(dA, len(A))
+    assert dev.info() == 0
+    assert np.max(np.abs(got - ref)) <= 1e-11
+
+
+@pytest.mark.parametrize("N,b", [(1024, 64), (1000, 64), (2048, 128), (1536, 256), (500, 20)])
+def test_cholesky_and_solve_match_oracle(hs, dev, oracle, N, b):
+    say("cholesky+solve N=%d b=%d" % (N, b))
+    X, A, rhs = rbf_system(oracle, N, 8, b)
+    Lref = oracle.cholesky(A, N, b)
+    xref = oracle.tri_solve(Lref, rhs, N, b)
+    dA, dx = dev.upload(A), dev.upload(rhs)
+    info = hs.call("hs_cholesky_f64", dev.h, dA.p, N, b)
+    assert info == 0
+    L = dev.download(dA, len(A))
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+    x = dev.download(dx, len(rhs))
+    # factor: two backward-stable FP64 factorisations with different
+    # summation orders differ by O(cond * u)
+    assert np.max(np.abs(L - Lref)) <= 1e-10
+    K = packed_to_dense(A, N, b)
+    rel_res = np.linalg.norm(K @ x[:N] - rhs[:N]) / np.linalg.norm(rhs[:N])
+    rel_diff = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    say("  rel residual %.3e  rel diff vs oracle %.3e" % (rel_res, rel_diff))
+    assert rel_res <= 1e-12
+    assert rel_diff <= 1e-9
+    assert np.all(x[N:] == 0.0)
+
+
+def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
+    N, b = 256, 64
+    K = np.eye(N)
+    K[130, 130] = -1.0
+    A = dense_to_packed(K, b)
+    dA = dev.upload(A)
+    info = hs.call("hs_cholesky_f64", dev.h, dA.p, N, b)
+    assert info == 131
+
+
+@pytest.mark.parametrize("N,b,eps", [(2048, 128, 1e-10), (3000, 64, 1e-10), (4096, 256, 1e-8)])
+def test_cg_matches_oracle(hs, dev, oracle, N, b, eps):
+    say("cg N=%d b=%d eps=%g" % (N, b, eps))
+    X, A, rhs = rbf_system(oracle, N, 8, b)
+    xref, itref, _ = oracle.cg(A, rhs, N, b, eps)
+    dA, drhs, dx = dev.upload(A), dev.upload(rhs), dev.zeros(len(rhs))
+    it, df, d0 = C.c_int64(), C.c_double(), C.c_double()
+    hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, eps, 100000, C.byref(it), C.byref(df), C.byref(d0))
+    x = dev.download(dx, len(rhs))
+    say("  iterations gpu %d oracle %d" % (it.value, itref))
+    assert abs(it.value - itref) <= max(1, int(0.05 * itref))
+    K = packed_to_dense(A, N, b)
+    assert np.linalg.norm(K @ x[:N] - rhs[:N]) <= 3 * eps * np.linalg.norm(rhs[:N])
+    assert df.value <= eps * eps * d0.value
+
+
+def test_cg_imax_and_zero_rhs(hs, dev, oracle):
+    N, b = 512, 64
+    _, A, rhs = rbf_system(oracle, N, 8, b)
+    dA, drhs, dx = dev.upload(A), dev.upload(rhs), dev.zeros(len(rhs))
+    it = C.c_int64()
+    hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-14, 7, C.byref(it), None, None)
+    assert it.value == 7
+    xref, itref, _ = oracle.cg(A, rhs, N, b, 1e-14, imax=7)
+    assert np.allclose(dev.download(dx, len(rhs)), xref, rtol=1e-9, atol=1e-12)
+    dz = dev.zeros(len(rhs))
+    hs.call("hs_cg_f64", dev.h, dA.p, dz.p, dx.p, N, b, 1e-10, 100, C.byref(it), None, None)
+    assert it.value == 0 and np.all(dev.download(dx, len(rhs)) == 0.0)
+
+
+@pytest.mark.parametrize("N,d,b,ell,sn2", [(1000, 8, 64, 1.0, 1e-2), (2048, 16, 256, 2.0, 1e-3),
+                                          (777, 8, 128, 1.0, 1e-2), (300, 5, 20, 0.8, 1e-2)])
+def test_rbf_assembly_matches_oracle(hs, dev, oracle, N, d, b, ell, sn2):
+    say("rbf N=%d d=%d b=%d" % (N, d, b))
+    X, _ = synth_gp_numpy(3, N, d)
+    ref = oracle.rbf(X, N, d, b, ell, 1.0, sn2)
+    dX, dA = dev.upload(X.reshape(-1)), dev.zeros(packed_len(N, b))
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, 1.0, sn2, dA.p)
+    got = dev.download(dA, packed_len(N, b))
+    assert np.max(np.abs(got - ref)) <= 4e-16 * np.maximum(1.0, np.abs(ref)).max() * 4
+
+
+def test_rbf_window_matches_oracle(hs, dev, oracle):
+    N, b = 1000, 64
+    t = np.sin(np.arange(N) * 0.01) + 0.05 * np.random.default_rng(0).standard_normal(N)
+    ref = np.zeros(packed_len(N, b))
+    oracle.or_rbf_window(dptr(t), N, b, dptr(ref))
+    dt, dA = dev.upload(t), dev.zeros(packed_len(N, b))
+    hs.call("hs_rbf_window_f64", dev.h, dt.p, N, b, dA.p)
+    assert np.max(np.abs(dev.download(dA, len(ref)) - ref)) <= 2e-15
