@@ -134,7 +134,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
             __syncthreads();
         }
     }
-    // epilogue: C -= acc; lane holds rows 4*lq + v, column lr of each 16x16 tile
+    // epilogue: C -= acc. For the f64 16x16x4 MFMA, register v of lane l holds
+    // row l/16 + 4v, column l%16 of the 16x16 tile (one row group per register,
+    // unlike the f32 variant's four consecutive rows).
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
 #pragma unroll
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
             const int col = wn0 + j * 16 + lr;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = wm0 + i * 16 + 4 * lq + v;
+                const int row = wm0 + i * 16 + lq + 4 * v;
                 if (diag && tm == tn && row < col) continue;
                 double* p = C + (i64)row * b + col;
                 *p = *p - acc[i][j][v];
