@@ -99,8 +99,13 @@ def cholesky_workload(hs, dev, syn, args):
     ph = np.array(phases).mean(axis=0)
     flops = N ** 3 / 3.0
     ms = float(np.median(step_ms))
-    upd_flops = (n - 1) * n * (2 * n - 1) / 6.0 * b ** 3  # sum_{m=1}^{n-1} m^2 b^3
     nl = int(launches[2])
+    # algorithmic flops of the timed trailing-update launches: step k updates
+    # m = n-k-1 block rows; its full update is m^2 b^3 flops (m SYRK blocks at
+    # b^3 + m(m-1)/2 GEMM blocks at 2 b^3). With the look-ahead, the main-stream
+    # launch of step k leaves out block column k+1 and covers (m-1)^2 b^3.
+    top = n - 1 if nl == n - 1 else n - 2
+    upd_flops = sum(j * j for j in range(1, top + 1)) * float(b) ** 3
     achieved = upd_flops / (ph[2] / 1e3) / 1e12
     return {
         "N": N, "b": b, "d": d, "seed": seed, "ell": ell, "sf2": sf2, "sn2": sn2,

@@ -16,6 +16,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep MFMA accumulators in ordinary VGPRs (no AGPR copies around loops)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
 
@@ -60,12 +62,13 @@ def build_host():
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
           "-o", tmp] + srcs + ["-L" + LIBDIR, "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
-    drv = os.path.join(HOST, "driver", "hs_driver.cpp")
-    if os.path.exists(drv):
-        exe = os.path.join(LIBDIR, "hs_driver")
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o", exe + ".tmp", drv,
-              "-L" + LIBDIR, "-lhs_host", "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
-        os.replace(exe + ".tmp", exe)
+    for src, name in ((os.path.join(HOST, "driver", "hs_driver.cpp"), "hs_driver"),
+                      (os.path.join(HOST, "tests", "hs_host_tests.cpp"), "hs_host_tests")):
+        if os.path.exists(src):
+            exe = os.path.join(LIBDIR, name)
+            _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o", exe + ".tmp", src,
+                  "-L" + LIBDIR, "-lhs_host", "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
+            os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -6,7 +6,7 @@
 //   pass 1, one work-group per stored block (I,J), in packed (memory) order:
 //     rowpart[ID] = A_IJ x_J           (b values)
 //     colpart[ID] = A_IJ^T x_I         (b values, I > J only)
-//   pass 2, one thread per row i of block row I, fixed order over J:
+//   pass 2, 64 rows of block row I per work-group, fixed order over J:
 //     y_i = sum_{J<=I} rowpart[ID(I,J)]_i + sum_{J>I} colpart[ID(J,I)]_i
 // Lanes read 16 contiguous bytes each, so one wave load covers 1 KB of a
 // block; row sums are finished by a recursive-halving shuffle reduction,
@@ -143,6 +143,11 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
 
 // pass 2 variants: 0 = y = sum; 1 = y = sum and dqpart[wg] = sum_i y_i d_i;
 // 2 = y = rhs - sum and part[wg] = sum_i y_i^2
+// One work-group per 64 consecutive rows (b is a multiple of 64, so they lie
+// in one block row I); wave w sums block columns [w n/4, (w+1) n/4) for its
+// lane's row, the four wave sums are added in wave order.
+constexpr int FIN_ROWS = 64;
+
 template <int MODE>
 __global__ __launch_bounds__(SYMV_THREADS) void symv_finish_kernel(const double* __restrict__ rowpart,
                                                                     const double* __restrict__ colpart, int n, int b,
@@ -151,29 +156,27 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_finish_kernel(const double*
                                                                     double* __restrict__ part,
                                                                     const i64* __restrict__ stop, i64 it) {
     if (stop && it > *stop) return;
-    __shared__ double red[SYMV_THREADS];
-    const i64 i = (i64)blockIdx.x * SYMV_THREADS + threadIdx.x;
-    const i64 rows = (i64)n * b;
-    double v = 0.0;
-    if (i < rows) {
-        const int I = (int)(i / b), ii = (int)(i % b);
-        double s = 0.0;
-        for (int J = 0; J <= I; ++J) s += rowpart[hs_block_id(I, J, n) * b + ii];
-        const i64 base = hs_block_id(I, I, n);
-        for (int J = I + 1; J < n; ++J) s += colpart[(base + (J - I)) * b + ii];
-        if (MODE == 2) s = other[i] - s;
-        y[i] = s;
-        if (MODE == 1) v = s * other[i];
-        if (MODE == 2) v = s * s;
-    }
-    if (MODE == 0) return;
-    red[threadIdx.x] = v;
+    __shared__ double red[4][FIN_ROWS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const i64 i = (i64)blockIdx.x * FIN_ROWS + lane;
+    const int I = (int)(i / b), ii = (int)(i % b);
+    const int j0 = (int)((i64)wave * n / 4), j1 = (int)((i64)(wave + 1) * n / 4);
+    double s = 0.0;
+    const int jr = min(j1, I + 1);
+    for (int J = j0; J < jr; ++J) s += rowpart[hs_block_id(I, J, n) * b + ii];
+    const i64 base = hs_block_id(I, I, n);
+    for (int J = max(j0, I + 1); J < j1; ++J) s += colpart[(base + (J - I)) * b + ii];
+    red[wave][lane] = s;
     __syncthreads();
-    for (int st = SYMV_THREADS / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if (wave != 0) return;
+    s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (MODE == 2) s = other[i] - s;
+    y[i] = s;
+    if (MODE == 0) return;
+    double v = (MODE == 1) ? s * other[i] : s * s;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == 0) part[blockIdx.x] = v;
 }
 
 // every work-group of the vector kernels sums the G partials in the same
@@ -323,7 +326,7 @@ struct SymvWork {
 static int symv_workspace(hs_handle_t h, int n, int b, bool withVectors, SymvWork* w) {
     const i64 nb = (i64)n * (n + 1) / 2;
     const i64 rows = (i64)n * b;
-    const i64 G = (rows + SYMV_THREADS - 1) / SYMV_THREADS;
+    const i64 G = rows / FIN_ROWS;  // the largest partial count (reduction pass grid)
     const size_t bytes = sizeof(double) * (size_t)(2 * nb * b + 2 * (G + 64) + (withVectors ? 3 * rows : 0)) +
                          sizeof(CGState) + 256;
     int rc = hs_ensure_work(h, bytes);
@@ -357,7 +360,7 @@ extern "C" int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* 
     if (rc) return rc;
     if ((rc = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, nullptr, 0))) return rc;
     const i64 rows = (i64)n * b;
-    const i64 G = (rows + SYMV_THREADS - 1) / SYMV_THREADS;
+    const i64 G = rows / FIN_ROWS;
     symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(w.rowpart, w.colpart, n, b, y,
                                                                                    nullptr, nullptr, nullptr, 0);
     HS_LAUNCH_CHECK();
@@ -383,7 +386,8 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
     if (imax < 0) return -8;
     const int n = (int)((N + b - 1) / b);
     const i64 rows = (i64)n * b;
-    const int G = (int)((rows + SYMV_THREADS - 1) / SYMV_THREADS);
+    const int G = (int)((rows + SYMV_THREADS - 1) / SYMV_THREADS);  // vector-kernel grid
+    const int GF = (int)(rows / FIN_ROWS);                          // reduction-pass grid
     SymvWork w;
     int rc = symv_workspace(h, n, b, true, &w);
     if (rc) return rc;
@@ -392,7 +396,7 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
     double* r = w.vec + 2 * rows;
     CGState* st = w.st;
     const i64* stop = &st->stop;
-    const dim3 vg((unsigned)G), vb(SYMV_THREADS);
+    const dim3 vg((unsigned)G), vb(SYMV_THREADS), fg((unsigned)GF);
 
     cg_init_kernel<<<vg, vb, 0, h->stream>>>(rhs, x, r, d, rows, w.part2);
     HS_LAUNCH_CHECK();
@@ -413,19 +417,20 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
             const i64 it = enq;
             int e;
             if ((e = launch_partials(h, A, d, w.rowpart, w.colpart, n, b, stop, it))) return e;
-            symv_finish_kernel<1><<<vg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, q, d, w.part1, stop, it);
+            symv_finish_kernel<1><<<fg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, q, d, w.part1, stop, it);
             HS_LAUNCH_CHECK();
             const int updateR = (it % 50) != 0;
-            cg_xr_kernel<<<vg, vb, 0, h->stream>>>(st, w.part1, G, d, q, x, r, rows, w.part2, it, updateR);
+            cg_xr_kernel<<<vg, vb, 0, h->stream>>>(st, w.part1, GF, d, q, x, r, rows, w.part2, it, updateR);
             HS_LAUNCH_CHECK();
             if (!updateR) {
                 // true residual every 50 iterations (CG.cpp:95-101)
                 if ((e = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, stop, it))) return e;
-                symv_finish_kernel<2><<<vg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, r, rhs, w.part2, stop,
+                symv_finish_kernel<2><<<fg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, r, rhs, w.part2, stop,
                                                                 it);
                 HS_LAUNCH_CHECK();
             }
-            cg_d_kernel<<<vg, vb, 0, h->stream>>>(st, w.part2, G, r, d, rows, it);
+            // r.r partials come from cg_xr (G of them) or from the residual pass (GF)
+            cg_d_kernel<<<vg, vb, 0, h->stream>>>(st, w.part2, updateR ? G : GF, r, d, rows, it);
             HS_LAUNCH_CHECK();
         }
         return HS_OK;

@@ -36,7 +36,18 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
         *J = *I;
         return;
     }
+    if (pm.mode == 5) {  // block column k+1 only (look-ahead panel update)
+        *I = pm.k + 1 + (int)q;
+        *J = pm.k + 1;
+        return;
+    }
     int r, c;
+    if (pm.mode == 4) {  // trailing triangle without block column k+1
+        hs_tri_decode(q, &r, &c);
+        *I = pm.k + 2 + r;
+        *J = pm.k + 2 + c;
+        return;
+    }
     hs_tri_decode(q + pm.pairOffset, &r, &c);
     if (pm.mode == 2) {
         *I = pm.k + 2 + r;
@@ -47,13 +58,16 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
     }
 }
 
-template <int TILE>
+// KIND only separates the look-ahead column launches (KIND 1) from the rest
+// in profiles; the code is the same.
+template <int TILE, int KIND>
 __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict__ A, int b, int n, PairMap pm) {
     constexpr int WT = TILE / 2;   // wave tile edge
     constexpr int MT = WT / 16;    // 16x16 MFMA tiles per wave edge
     constexpr int LDP = TILE + 4;  // LDS pitch (doubles) of one k row
-    __shared__ double As[KT * LDP];
-    __shared__ double Bs[KT * LDP];
+    constexpr int STAGE = KT * LDP;
+    __shared__ double As[2 * STAGE];
+    __shared__ double Bs[2 * STAGE];
 
     const int tilesPerEdge = b / TILE;
     const int tilesPerBlock = tilesPerEdge * tilesPerEdge;
@@ -80,7 +94,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
     constexpr int TPR = KT / LPT;             // threads per row
     const int lrow = tid / TPR;
     const int lk = (tid % TPR) * LPT;
-    double ra[LPT], rb[LPT];
+    double2 ra[LPT / 2], rb[LPT / 2];
+    const double* pa = La + (i64)lrow * b + lk;
+    const double* pb = Lb + (i64)lrow * b + lk;
 
     d4 acc[MT][MT];
 #pragma unroll
@@ -90,64 +106,79 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 
     const int nk = b / KT;
     auto gload = [&](int kt) {
-        const double* pa = La + (i64)lrow * b + kt * KT + lk;
-        const double* pb = Lb + (i64)lrow * b + kt * KT + lk;
 #pragma unroll
-        for (int v = 0; v < LPT; v += 2) {
-            const double2 x = *reinterpret_cast<const double2*>(pa + v);
-            const double2 y = *reinterpret_cast<const double2*>(pb + v);
-            ra[v] = x.x;
-            ra[v + 1] = x.y;
-            rb[v] = y.x;
-            rb[v + 1] = y.y;
+        for (int v = 0; v < LPT / 2; ++v) {
+            ra[v] = *reinterpret_cast<const double2*>(pa + kt * KT + 2 * v);
+            rb[v] = *reinterpret_cast<const double2*>(pb + kt * KT + 2 * v);
         }
     };
-    auto sstore = [&]() {
+    auto sstore = [&](int buf) {
+        double* as = As + buf * STAGE;
+        double* bs = Bs + buf * STAGE;
 #pragma unroll
-        for (int v = 0; v < LPT; ++v) {
-            As[(lk + v) * LDP + lrow] = ra[v];
-            Bs[(lk + v) * LDP + lrow] = rb[v];
+        for (int v = 0; v < LPT / 2; ++v) {
+            as[(lk + 2 * v) * LDP + lrow] = ra[v].x;
+            as[(lk + 2 * v + 1) * LDP + lrow] = ra[v].y;
+            bs[(lk + 2 * v) * LDP + lrow] = rb[v].x;
+            bs[(lk + 2 * v + 1) * LDP + lrow] = rb[v].y;
         }
     };
 
+    // pipeline: stage kt computes from LDS buffer kt&1 while stage kt+1 sits
+    // in the other buffer and stage kt+2 is in flight from global memory
     gload(0);
-    sstore();
+    sstore(0);
+    if (nk > 1) gload(1);
     __syncthreads();
     const int lr = lane & 15, lq = lane >> 4;
     for (int kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) gload(kt + 1);
+        const double* as = As + (kt & 1) * STAGE;
+        const double* bs = Bs + (kt & 1) * STAGE;
 #pragma unroll
         for (int ks = 0; ks < KT; ks += 4) {
             double av[MT], bv[MT];
 #pragma unroll
-            for (int i = 0; i < MT; ++i) av[i] = As[(ks + lq) * LDP + wm0 + i * 16 + lr];
+            for (int i = 0; i < MT; ++i) av[i] = as[(ks + lq) * LDP + wm0 + i * 16 + lr];
 #pragma unroll
-            for (int j = 0; j < MT; ++j) bv[j] = Bs[(ks + lq) * LDP + wn0 + j * 16 + lr];
+            for (int j = 0; j < MT; ++j) bv[j] = bs[(ks + lq) * LDP + wn0 + j * 16 + lr];
 #pragma unroll
             for (int i = 0; i < MT; ++i)
 #pragma unroll
-                for (int j = 0; j < MT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < MT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) {
+            sstore((kt + 1) & 1);
+            if (kt + 2 < nk) gload(kt + 2);
         }
         __syncthreads();
-        if (kt + 1 < nk) {
-            sstore();
-            __syncthreads();
-        }
     }
     // epilogue: C -= acc. For the f64 16x16x4 MFMA, register v of lane l holds
     // row l/16 + 4v, column l%16 of the 16x16 tile (one row group per register,
     // unlike the f32 variant's four consecutive rows).
+    if (!(diag && tm == tn)) {
 #pragma unroll
-    for (int i = 0; i < MT; ++i) {
+        for (int i = 0; i < MT; ++i) {
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-            const int col = wn0 + j * 16 + lr;
+            for (int j = 0; j < MT; ++j) {
+                double* p = C + (i64)(wm0 + i * 16 + lq) * b + wn0 + j * 16 + lr;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = wm0 + i * 16 + lq + 4 * v;
-                if (diag && tm == tn && row < col) continue;
-                double* p = C + (i64)row * b + col;
-                *p = *p - acc[i][j][v];
+                for (int v = 0; v < 4; ++v) p[(i64)4 * v * b] -= acc[i][j][v];
+            }
+        }
+    } else {
+        // diagonal tile of a diagonal block: lower triangle only (the
+        // reference's SYRK writes i >= j only, MatrixMatrixOperations.cpp:273-299)
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j) {
+                const int col = wn0 + j * 16 + lr;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = wm0 + i * 16 + lq + 4 * v;
+                    if (row >= col) C[(i64)row * b + col] -= acc[i][j][v];
+                }
             }
         }
     }
@@ -158,9 +189,16 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 // Dispatch: MFMA kernel when b is a multiple of 64 (the reference's optimised
 // GEMM has the same constraint, MatrixMatrixOperations.cpp:657-659), else the
 // element-wise kernel.
-int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode) {
+int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
+                          int mode) {
     if (rowCount <= 0) return HS_OK;
-    if (b % 64 != 0) return hs_launch_trailing_generic(h, A, b, k, rowStart, rowCount, n, mode);
+    if (b % 64 != 0) {
+        if (mode > 3) {
+            hs_set_error("look-ahead trailing modes need b % 64 == 0");
+            return HS_ERR_UNSUPPORTED;
+        }
+        return hs_launch_trailing_generic(h, A, b, k, rowStart, rowCount, n, mode);
+    }
     const int tile = (b % 128 == 0) ? 128 : 64;
     const i64 tilesPerBlock = (i64)(b / tile) * (b / tile);
     auto launch = [&](int md, i64 pairs, i64 off, int rs) -> int {
@@ -171,14 +209,23 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
             hs_set_error("trailing update grid too large");
             return HS_ERR_UNSUPPORTED;
         }
-        if (tile == 128)
-            trailing_mfma_f64<128><<<dim3((unsigned)grid), dim3(THREADS), 0, h->stream>>>(A, b, n, pm);
-        else
-            trailing_mfma_f64<64><<<dim3((unsigned)grid), dim3(THREADS), 0, h->stream>>>(A, b, n, pm);
+        const dim3 g((unsigned)grid), t(THREADS);
+        if (tile == 128) {
+            if (md == 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+            else trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+        } else {
+            if (md == 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+            else trailing_mfma_f64<64, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+        }
         HS_LAUNCH_CHECK();
         return HS_OK;
     };
     if (mode == 1) return launch(1, rowCount, 0, rowStart);
+    if (mode == 4) {  // rows k+2..n-1, columns k+2..I
+        const i64 m = n - k - 2;
+        return launch(4, m * (m + 1) / 2, 0, k + 2);
+    }
+    if (mode == 5) return launch(5, n - k - 1, 0, k + 1);  // column k+1, rows k+1..n-1
     if (mode == 2) {
         const int firstRow = std::max(rowStart, k + 2);
         const int lastRow = rowStart + rowCount;
@@ -191,35 +238,185 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
     return launch(3, r1 * (r1 + 1) / 2 - r0 * (r0 + 1) / 2, r0 * (r0 + 1) / 2, rowStart);
 }
 
+int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode) {
+    return hs_launch_trailing_on(h, h->stream, A, b, k, rowStart, rowCount, n, mode);
+}
+
+namespace {
+
+// Runs launches that use h->stream on another stream of the same handle.
+struct OnStream {
+    hs_handle_t h;
+    hipStream_t saved;
+    OnStream(hs_handle_t hh, hipStream_t s) : h(hh), saved(hh->stream) { h->stream = s; }
+    ~OnStream() { h->stream = saved; }
+};
+
+// Per-phase event bracketing for the timed variant: [start, end) pairs on a
+// stream, summed after the final synchronisation.
+struct PhaseTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+    bool on = false;
+    int begin(hipStream_t s, int phase, hipEvent_t* out) {
+        if (!on) return HS_OK;
+        hipEvent_t e;
+        HS_HIP(hipEventCreate(&e));
+        HS_HIP(hipEventRecord(e, s));
+        spans[phase].push_back({e, nullptr});
+        *out = e;
+        return HS_OK;
+    }
+    int end(hipStream_t s, int phase) {
+        if (!on) return HS_OK;
+        hipEvent_t e;
+        HS_HIP(hipEventCreate(&e));
+        HS_HIP(hipEventRecord(e, s));
+        spans[phase].back().second = e;
+        return HS_OK;
+    }
+    int collect(double* ms, int64_t* launches) {
+        for (int p = 0; p < 3; ++p) {
+            double t = 0;
+            for (auto& pr : spans[p]) {
+                float v = 0;
+                HS_HIP(hipEventElapsedTime(&v, pr.first, pr.second));
+                t += v;
+            }
+            ms[p] = t;
+            launches[p] = (int64_t)spans[p].size();
+        }
+        return HS_OK;
+    }
+    ~PhaseTimer() {
+        for (auto& sp : spans)
+            for (auto& pr : sp) {
+                if (pr.first) hipEventDestroy(pr.first);
+                if (pr.second) hipEventDestroy(pr.second);
+            }
+    }
+};
+
+// Right-looking factorisation with a look-ahead of one panel on two streams
+// (same priority). Stream S2 factors panels: it applies step k's update to
+// block column k+1 only, then POTRF and TRSM of panel k+1. Stream S1 applies
+// step k's update to the rest of the trailing triangle (J >= k+2) as soon as
+// panel k is final, so POTRF/TRSM of k+1 run underneath it.
+// Dependencies: S1 update k waits for panel k (event P[k]); S2's column update
+// of step k+1 waits for S1's update k (it also wrote block column k+2).
+int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
+    const int n = (int)((N + b - 1) / b);
+    int rc = hs_reset_info(h);
+    if (rc) return rc;
+    rc = hs_ensure_events(h, 2 * n + 2);
+    if (rc) return rc;
+    hipEvent_t* evP = h->events.data();          // panel k final (S2)
+    hipEvent_t* evU = h->events.data() + n + 1;  // S1 update k done
+    hipStream_t s1 = h->stream, s2 = h->stream2;
+    hipEvent_t dummy;
+    // S2 must see everything already queued on S1 (e.g. the assembly of A)
+    HS_HIP(hipEventRecord(evU[n], s1));
+    HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
+    {
+        OnStream os(h, s2);
+        if ((rc = pt.begin(s2, 0, &dummy))) return rc;
+        if ((rc = hs_launch_potrf(h, A, N, b, hs_block_id(0, 0, n), 0))) return rc;
+        if ((rc = pt.end(s2, 0))) return rc;
+        if (n > 1) {
+            if ((rc = pt.begin(s2, 1, &dummy))) return rc;
+            if ((rc = hs_launch_trsm(h, A, b, hs_block_id(0, 0, n), 0, 1, n - 1))) return rc;
+            if ((rc = pt.end(s2, 1))) return rc;
+        }
+    }
+    HS_HIP(hipEventRecord(evP[0], s2));
+    for (int k = 0; k + 1 < n; ++k) {
+        const int m = n - k - 1;  // blocks below the diagonal in column k
+        // S2: column k+1 update with panel k, then panel k+1
+        if (k >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[k - 1], 0));
+        if ((rc = hs_launch_trailing_on(h, s2, A, b, k, k + 1, m, n, 5))) return rc;
+        {
+            OnStream os(h, s2);
+            const i64 id = hs_block_id(k + 1, k + 1, n);
+            if ((rc = pt.begin(s2, 0, &dummy))) return rc;
+            if ((rc = hs_launch_potrf(h, A, N, b, id, k + 1))) return rc;
+            if ((rc = pt.end(s2, 0))) return rc;
+            if (m > 1) {
+                if ((rc = pt.begin(s2, 1, &dummy))) return rc;
+                if ((rc = hs_launch_trsm(h, A, b, id, k + 1, k + 2, m - 1))) return rc;
+                if ((rc = pt.end(s2, 1))) return rc;
+            }
+        }
+        HS_HIP(hipEventRecord(evP[k + 1], s2));
+        // S1: the rest of step k's trailing update
+        HS_HIP(hipStreamWaitEvent(s1, evP[k], 0));
+        if (m > 1) {
+            if ((rc = pt.begin(s1, 2, &dummy))) return rc;
+            if ((rc = hs_launch_trailing_on(h, s1, A, b, k, k + 2, m - 1, n, 4))) return rc;
+            if ((rc = pt.end(s1, 2))) return rc;
+        }
+        HS_HIP(hipEventRecord(evU[k], s1));
+    }
+    // join: S1 waits for the last panel
+    HS_HIP(hipStreamWaitEvent(s1, evP[n - 1], 0));
+    return HS_OK;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Whole factorisation, Cholesky::solve_heterogeneous at share 1.0
 // (Cholesky.cpp:446-476): per column k, POTRF of the diagonal block, TRSM of
-// the n-k-1 blocks below it, then the fused SYRK+GEMM trailing update.
-// Everything is stream ordered; the only host synchronisation is the final
-// read of the pivot status.
+// the n-k-1 blocks below it, then the SYRK+GEMM trailing update. For b a
+// multiple of 64 the look-ahead driver above runs the panel work of step k+1
+// underneath the trailing update of step k; other block sizes run the plain
+// sequence on one stream. Everything is stream ordered; the only host
+// synchronisation is the final read of the pivot status.
 // ---------------------------------------------------------------------------
+static int cholesky_sequential(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
+    const int n = (int)((N + b - 1) / b);
+    int rc = hs_reset_info(h);
+    if (rc) return rc;
+    hipEvent_t dummy;
+    for (int k = 0; k < n; ++k) {
+        const i64 id = hs_block_id(k, k, n);
+        if ((rc = pt.begin(h->stream, 0, &dummy))) return rc;
+        if ((rc = hs_launch_potrf(h, A, N, b, id, k))) return rc;
+        if ((rc = pt.end(h->stream, 0))) return rc;
+        const int cnt = n - k - 1;
+        if (cnt > 0) {
+            if ((rc = pt.begin(h->stream, 1, &dummy))) return rc;
+            if ((rc = hs_launch_trsm(h, A, b, id, k, k + 1, cnt))) return rc;
+            if ((rc = pt.end(h->stream, 1))) return rc;
+            if ((rc = pt.begin(h->stream, 2, &dummy))) return rc;
+            if ((rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) return rc;
+            if ((rc = pt.end(h->stream, 2))) return rc;
+        }
+    }
+    return HS_OK;
+}
+
+static int cholesky_run(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
+    int rc = (b % 64 == 0) ? cholesky_lookahead(h, A, N, b, pt) : cholesky_sequential(h, A, N, b, pt);
+    if (rc) {
+        hipStreamSynchronize(h->stream2);
+        return rc;
+    }
+    int64_t info = 0;
+    if ((rc = hs_get_info(h, &info))) return rc;  // synchronises the handle's stream
+    return (int)info;
+}
+
 extern "C" int hs_cholesky_f64(hs_handle_t h, double* A, int64_t N, int b) {
     if (!h) return -1;
     if (!A) return -2;
     if (N <= 0) return -3;
     if (b <= 0) return -4;
-    const int n = (int)((N + b - 1) / b);
-    int rc = hs_reset_info(h);
-    if (rc) return rc;
-    for (int k = 0; k < n; ++k) {
-        const i64 id = hs_block_id(k, k, n);
-        if ((rc = hs_launch_potrf(h, A, N, b, id, k))) return rc;
-        const int cnt = n - k - 1;
-        if (cnt > 0) {
-            if ((rc = hs_launch_trsm(h, A, b, id, k, k + 1, cnt))) return rc;
-            if ((rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) return rc;
-        }
-    }
-    int64_t info = 0;
-    if ((rc = hs_get_info(h, &info))) return rc;
-    return (int)info;
+    PhaseTimer pt;
+    return cholesky_run(h, A, N, b, pt);
 }
 
+// phase 0 = POTRF launches, 1 = TRSM launches, 2 = trailing-update launches
+// on the main stream (the look-ahead column updates run on the panel stream
+// and are not counted in phase 2).
 extern "C" int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* phase_ms,
                                      int64_t* launches) {
     if (!h) return -1;
@@ -228,42 +425,12 @@ extern "C" int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b,
     if (b <= 0) return -4;
     if (!phase_ms) return -5;
     if (!launches) return -6;
-    const int n = (int)((N + b - 1) / b);
-    int rc = hs_reset_info(h);
-    if (rc) return rc;
-    // events: before POTRF, before TRSM, before update, after update, per step
-    std::vector<hipEvent_t> ev((size_t)4 * n);
-    for (auto& e : ev) HS_HIP(hipEventCreate(&e));
-    for (int k = 0; k < n; ++k) {
-        const i64 id = hs_block_id(k, k, n);
-        const int cnt = n - k - 1;
-        HS_HIP(hipEventRecord(ev[4 * k], h->stream));
-        if ((rc = hs_launch_potrf(h, A, N, b, id, k))) break;
-        HS_HIP(hipEventRecord(ev[4 * k + 1], h->stream));
-        if (cnt > 0 && (rc = hs_launch_trsm(h, A, b, id, k, k + 1, cnt))) break;
-        HS_HIP(hipEventRecord(ev[4 * k + 2], h->stream));
-        if (cnt > 0 && (rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) break;
-        HS_HIP(hipEventRecord(ev[4 * k + 3], h->stream));
-    }
-    if (rc == 0) {
-        HS_HIP(hipStreamSynchronize(h->stream));
-        double t[3] = {0, 0, 0};
-        for (int k = 0; k < n; ++k)
-            for (int p = 0; p < 3; ++p) {
-                float ms = 0;
-                HS_HIP(hipEventElapsedTime(&ms, ev[4 * k + p], ev[4 * k + p + 1]));
-                t[p] += ms;
-            }
-        for (int p = 0; p < 3; ++p) phase_ms[p] = t[p];
-        launches[0] = n;
-        launches[1] = n - 1;
-        launches[2] = n - 1;
-    }
-    for (auto& e : ev) hipEventDestroy(e);
-    if (rc) return rc;
-    int64_t info = 0;
-    if ((rc = hs_get_info(h, &info))) return rc;
-    return (int)info;
+    PhaseTimer pt;
+    pt.on = true;
+    int rc = cholesky_run(h, A, N, b, pt);
+    if (rc < 0) return rc;
+    int rc2 = pt.collect(phase_ms, launches);
+    return rc2 ? rc2 : rc;
 }
 
 // Forward and backward blocked substitution (TriangularSystemSolver.cpp:31-82).

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "hs_abi.h"
 
@@ -13,7 +14,9 @@ typedef int64_t i64;
 // Scratch sizes are fixed per handle once grown; kernels never allocate.
 struct hs_handle_s {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the handle's ordering stream
+    hipStream_t stream2 = nullptr;  // look-ahead panel stream (same priority)
+    std::vector<hipEvent_t> events; // reusable ordering events (no timing)
     int* info_dev = nullptr;   // sticky first bad pivot (1-based global row), 0 = none
     // grow-only device workspace for partial sums / scalars
     void* work = nullptr;
@@ -32,6 +35,8 @@ int hs_hip_fail(hipError_t e, const char* what);
 // Ensures the handle workspace holds at least `bytes`; called only from
 // non-hot set-up points (first call of an entry on a size).
 int hs_ensure_work(hs_handle_t h, size_t bytes);
+// Ensures the handle's event pool holds at least `count` events.
+int hs_ensure_events(hs_handle_t h, size_t count);
 
 #define HS_HIP(call)                                                \
     do {                                                            \
@@ -60,6 +65,10 @@ int hs_launch_trsm(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, i
 // trailing update over rows [rowStart, rowStart+rowCount) of step k;
 // mode 1 = diagonal blocks only (SYRK), 2 = off-diagonal only (GEMM), 3 = both
 int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode);
+// same on an explicit stream; modes 4 (J >= k+2 rows k+2..n-1) and 5 (block
+// column k+1, rows k+1..n-1) split step k for the look-ahead driver
+int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
+                          int mode);
 int hs_launch_trsv_block(hs_handle_t h, const double* A, double* x, i64 N, int b, int blockRow, i64 blockID, int transposed);
 int hs_launch_col_update(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
                          i64 blockID, int n, int transposed);

@@ -36,7 +36,9 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
     hs_handle_s* h = new hs_handle_s();
     h->device = device;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (h->stream) hipStreamDestroy(h->stream);
         delete h;
         return hs_hip_fail(e, "hipStreamCreateWithFlags");
     }
@@ -58,6 +60,9 @@ extern "C" int hs_handle_destroy(hs_handle_t h) {
     if (!h) return -1;
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
+    hipStreamSynchronize(h->stream2);
+    for (auto e : h->events) hipEventDestroy(e);
+    hipStreamDestroy(h->stream2);
     if (h->work) hipFree(h->work);
     if (h->info_dev) hipFree(h->info_dev);
     if (h->pinned) hipHostFree(h->pinned);
@@ -90,6 +95,15 @@ int hs_ensure_work(hs_handle_t h, size_t bytes) {
     return HS_OK;
 }
 
+int hs_ensure_events(hs_handle_t h, size_t count) {
+    while (h->events.size() < count) {
+        hipEvent_t e;
+        HS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->events.push_back(e);
+    }
+    return HS_OK;
+}
+
 extern "C" int hs_malloc(void** ptr, size_t bytes) {
     if (!ptr) return -1;
     hipError_t e = hipMalloc(ptr, bytes == 0 ? 1 : bytes);
@@ -103,6 +117,31 @@ extern "C" int hs_malloc(void** ptr, size_t bytes) {
 
 extern "C" int hs_free(void* ptr) {
     if (ptr) HS_HIP(hipFree(ptr));
+    return HS_OK;
+}
+
+extern "C" int hs_host_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return -1;
+    hipError_t e = hipHostMalloc(ptr, bytes == 0 ? 1 : bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        hs_hip_fail(e, "hipHostMalloc");
+        return HS_ERR_NOMEM;
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_host_free(void* ptr) {
+    if (ptr) HS_HIP(hipHostFree(ptr));
+    return HS_OK;
+}
+
+extern "C" int hs_device_memory(hs_handle_t h, size_t* free_bytes, size_t* total_bytes) {
+    if (!h) return -1;
+    if (!free_bytes) return -2;
+    if (!total_bytes) return -3;
+    HS_HIP(hipSetDevice(h->device));
+    HS_HIP(hipMemGetInfo(free_bytes, total_bytes));
     return HS_OK;
 }
 

@@ -1,0 +1,175 @@
+// CG, Cholesky and TriangularSystemSolver of the host API over the C ABI.
+// The reference splits rows between a CPU and a GPU queue by the load
+// balancer's share; this build runs the GPU share 1.0 only and rejects any
+// other share with std::invalid_argument.
+#include <chrono>
+#include <ctime>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+
+#include "hs/hs.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+std::string timeString() {
+    std::time_t t = std::time(nullptr);
+    std::tm tm = *std::localtime(&t);
+    std::ostringstream s;
+    s << std::put_time(&tm, "%Y_%m_%d__%H_%M_%S");
+    return s.str();
+}
+
+void requireFullGPUShare(const LoadBalancer& lb, const char* who) {
+    if (lb.currentProportionGPU != 1.0) {
+        throw std::invalid_argument(std::string(who) +
+                                    ": this build runs at GPU share 1.0 only (heterogeneous CPU+GPU splitting is not "
+                                    "part of it); got share " +
+                                    std::to_string(lb.currentProportionGPU));
+    }
+}
+
+template <typename V>
+void writeResult(const std::string& dir, const V& x) {
+    std::ofstream out(dir + "/x_result.txt");
+    out << std::setprecision(20) << std::scientific;
+    for (std::size_t i = 0; i < x.size(); ++i) out << x[i] << "\n";
+}
+
+}  // namespace
+
+CG::CG(SymmetricMatrix& A, RightHandSide& b, hs::Queue& cpuQueue, hs::Queue& gpuQueue,
+       std::shared_ptr<LoadBalancer> loadBalancer)
+    : A(A), b(b), cpuQueue(cpuQueue), gpuQueue(gpuQueue), loadBalancer(std::move(loadBalancer)) {
+    if (A.N != b.N) {
+        throw std::invalid_argument("Dimensions of A and b do not match: " + std::to_string(A.N) +
+                                    " != " + std::to_string(b.N));
+    }
+    if (A.blockCountXY != b.blockCountX) {
+        throw std::invalid_argument("Block count of A and b do not match: " + std::to_string(A.blockCountXY) +
+                                    " != " + std::to_string(b.blockCountX));
+    }
+    x.assign(b.rightHandSideData.size(), 0.0);
+}
+
+void CG::solveHeterogeneous() {
+    const auto start = clk::now();
+    metricsTracker.startTracking();
+    requireFullGPUShare(*loadBalancer, "CG");
+    hs_handle_t h = gpuQueue.handle();
+    const std::size_t bytesA = A.matrixData.size() * sizeof(double);
+    const std::size_t bytesV = b.rightHandSideData.size() * sizeof(double);
+    if (bytesA + 6 * bytesV > static_cast<std::size_t>(0.9 * static_cast<double>(gpuQueue.global_mem_size()))) {
+        throw std::runtime_error("GPU memory not sufficient for the matrix (this build keeps it whole on the GPU)");
+    }
+    auto t0 = clk::now();
+    hs::DeviceArray<double> dA(A.matrixData.size()), db(b.rightHandSideData.size()), dx(b.rightHandSideData.size());
+    hs::check(hs_memcpy_htod(h, dA.data(), A.matrixData.data(), bytesA), "hs_memcpy_htod");
+    hs::check(hs_memcpy_htod(h, db.data(), b.rightHandSideData.data(), bytesV), "hs_memcpy_htod");
+    metricsTracker.memoryInitTime = ms_since(t0);
+
+    t0 = clk::now();
+    int64_t it = 0;
+    double df = 0.0, d0 = 0.0;
+    hs::check(hs_cg_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize, conf::epsilon,
+                        static_cast<int64_t>(conf::iMax), &it, &df, &d0),
+              "hs_cg_f64");
+    const double solveMs = ms_since(t0);
+    iterations = static_cast<std::size_t>(it);
+    deltaZero = d0;
+    deltaFinal = df;
+    // the device loop has no per-iteration host timing: record the mean
+    for (std::size_t i = 0; i < iterations; ++i)
+        metricsTracker.updateMetrics(i, A.blockCountXY, 0, solveMs / static_cast<double>(iterations),
+                                     loadBalancer->updateInterval);
+
+    t0 = clk::now();
+    hs::check(hs_memcpy_dtoh(h, x.data(), dx.data(), bytesV), "hs_memcpy_dtoh");
+    metricsTracker.resultCopyTime = ms_since(t0);
+    metricsTracker.totalTime = ms_since(start);
+    std::cout << "Total time: " << metricsTracker.totalTime << "ms (" << iterations << " iterations)" << std::endl;
+    metricsTracker.endTracking();
+    std::string dir = conf::outputPath + "/" + timeString();
+    std::filesystem::create_directories(dir);
+    metricsTracker.writeJSON(dir);
+    if (conf::writeResult) writeResult(".", x);
+}
+
+Cholesky::Cholesky(SymmetricMatrix& A, hs::Queue& cpuQueue, hs::Queue& gpuQueue,
+                   std::shared_ptr<LoadBalancer> loadBalancer)
+    : A(A), cpuQueue(cpuQueue), gpuQueue(gpuQueue), loadBalancer(std::move(loadBalancer)) {}
+
+Cholesky::~Cholesky() {
+    if (A_gpu) hs_free(A_gpu);
+}
+
+void Cholesky::solve_heterogeneous() {
+    const auto start = clk::now();
+    metricsTracker.startTracking();
+    requireFullGPUShare(*loadBalancer, "Cholesky");
+    hs_handle_t h = gpuQueue.handle();
+    const std::size_t bytes = A.matrixData.size() * sizeof(double);
+    auto t0 = clk::now();
+    if (!A_gpu) {
+        void* p = nullptr;
+        hs::check(hs_malloc(&p, bytes), "hs_malloc");
+        A_gpu = static_cast<double*>(p);
+    }
+    hs::check(hs_memcpy_htod(h, A_gpu, A.matrixData.data(), bytes), "hs_memcpy_htod");
+    metricsTracker.memoryInitTime = ms_since(t0);
+
+    double phase[3] = {0, 0, 0};
+    int64_t launches[3] = {0, 0, 0};
+    const int rc = hs_cholesky_f64_timed(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize, phase, launches);
+    hs::check(rc, "hs_cholesky_f64");
+    info = rc;
+    if (info > 0) std::cerr << "warning: non-positive pivot at row " << info << " (matrix not SPD)" << std::endl;
+    metricsTracker.choleskyDiagonalBlockTimes.push_back(phase[0]);
+    metricsTracker.triangularSolveTimes_GPU.push_back(phase[1]);
+    metricsTracker.matrixMatrixTimes_GPU.push_back(phase[2]);
+
+    t0 = clk::now();
+    hs::check(hs_memcpy_dtoh(h, A.matrixData.data(), A_gpu, bytes), "hs_memcpy_dtoh");
+    metricsTracker.resultCopyTime = ms_since(t0);
+    metricsTracker.totalTime = ms_since(start);
+    std::cout << "Total time:  " << metricsTracker.totalTime << "ms" << std::endl;
+    if (!conf::trackCholeskySolveStep) metricsTracker.endTracking();
+    if (conf::writeMatrix) MatrixParser::writeFullMatrix("./A_chol_result", A);
+}
+
+void Cholesky::writeMetricsToFile() {
+    std::string dir = conf::outputPath + "/" + timeString();
+    std::filesystem::create_directories(dir);
+    metricsTracker.writeJSON(dir);
+}
+
+TriangularSystemSolver::TriangularSystemSolver(SymmetricMatrix& A, conf::fp_type* A_gpu, RightHandSide& b,
+                                               hs::Queue& cpuQueue, hs::Queue& gpuQueue,
+                                               std::shared_ptr<LoadBalancer> loadBalancer)
+    : A(A), b(b), A_gpu(A_gpu), cpuQueue(cpuQueue), gpuQueue(gpuQueue), loadBalancer(std::move(loadBalancer)) {}
+
+double TriangularSystemSolver::solve() {
+    const auto start = clk::now();
+    // the reference solves on the GPU only when the share is exactly 1
+    // (TriangularSystemSolver.cpp:18)
+    if (conf::initialProportionGPU != 1.0) {
+        throw std::invalid_argument("TriangularSystemSolver: this build solves on the GPU at share 1.0 only");
+    }
+    if (!A_gpu) throw std::invalid_argument("TriangularSystemSolver: no device factor (run Cholesky first)");
+    hs_handle_t h = gpuQueue.handle();
+    const std::size_t bytes = b.rightHandSideData.size() * sizeof(double);
+    hs::DeviceArray<double> db(b.rightHandSideData.size());
+    hs::check(hs_memcpy_htod(h, db.data(), b.rightHandSideData.data(), bytes), "hs_memcpy_htod");
+    hs::check(hs_tri_solve_f64(h, A_gpu, db.data(), static_cast<int64_t>(A.N), A.blockSize), "hs_tri_solve_f64");
+    hs::check(hs_memcpy_dtoh(h, b.rightHandSideData.data(), db.data(), bytes), "hs_memcpy_dtoh");
+    const double ms = ms_since(start);
+    std::cout << "Time to solve the triangular system: " << ms << "ms" << std::endl;
+    if (conf::writeResult) writeResult(".", b.rightHandSideData);
+    return ms;
+}

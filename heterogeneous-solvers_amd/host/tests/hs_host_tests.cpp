@@ -1,0 +1,79 @@
+// Probe program for the C++ host API, driven by tests/test_host_api.py.
+// Subcommands print what the pytest side compares with the reference's
+// golden values (no GPU needed for any of them):
+//   layout <matrix.txt> <bsz> <idx>...   values of A.matrixData at idx
+//   shape <matrix.txt> <bsz>             blockSize blockCountXY size
+//   rowparse <string>                    parsed values or invalid_argument
+//   rhs <vector.txt> <bsz>               padded right-hand side
+//   lb <share>                           StaticLoadBalancer validation
+//   cgdims                               CG dimension checks
+#include <iomanip>
+#include <iostream>
+#include <string>
+
+#include "hs/hs.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    std::string cmd = argv[1];
+    std::cout << std::setprecision(17);
+    hs::Queue host(hs::cpu_selector_v);
+    try {
+        if (cmd == "layout" || cmd == "shape") {
+            std::string path = argv[2];
+            conf::matrixBlockSize = std::stoul(argv[3]);
+            SymmetricMatrix A = MatrixParser::parseSymmetricMatrix(path, host);
+            if (cmd == "shape") {
+                std::cout << A.blockSize << " " << A.blockCountXY << " " << A.matrixData.size() << std::endl;
+                return 0;
+            }
+            for (int i = 4; i < argc; ++i) std::cout << A.matrixData.at(std::stoul(argv[i])) << "\n";
+            return 0;
+        }
+        if (cmd == "rowparse") {
+            try {
+                for (double v : MatrixParser::getRowValuesFromString(argv[2])) std::cout << v << "\n";
+            } catch (const std::invalid_argument&) {
+                std::cout << "invalid_argument" << std::endl;
+            }
+            return 0;
+        }
+        if (cmd == "rhs") {
+            std::string path = argv[2];
+            conf::matrixBlockSize = std::stoul(argv[3]);
+            RightHandSide b = MatrixParser::parseRightHandSide(path, host);
+            for (double v : b.rightHandSideData) std::cout << v << "\n";
+            return 0;
+        }
+        if (cmd == "lb") {
+            try {
+                StaticLoadBalancer lb(10, std::stod(argv[2]), 4);
+                MetricsTracker m;
+                std::cout << "ok " << lb.getNewProportionGPU(m) << std::endl;
+            } catch (const std::runtime_error& e) {
+                std::cout << "runtime_error " << e.what() << std::endl;
+            }
+            return 0;
+        }
+        if (cmd == "cgdims") {
+            conf::matrixBlockSize = 4;
+            SymmetricMatrix A(20, 4, host);
+            RightHandSide b(21, 4, host);
+            RightHandSide c(20, 5, host);
+            auto lb = std::make_shared<StaticLoadBalancer>(10, 1.0, A.blockCountXY);
+            for (RightHandSide* r : {&b, &c}) {
+                try {
+                    CG cg(A, *r, host, host, lb);
+                    std::cout << "no error" << std::endl;
+                } catch (const std::invalid_argument& e) {
+                    std::cout << "invalid_argument " << e.what() << std::endl;
+                }
+            }
+            return 0;
+        }
+    } catch (const std::exception& e) {
+        std::cout << "error " << e.what() << std::endl;
+        return 1;
+    }
+    return 2;
+}

@@ -31,6 +31,9 @@ _SIGS = {
     "hs_device_count": [C.POINTER(_I)],
     "hs_malloc": [C.POINTER(_P), C.c_size_t],
     "hs_free": [_P],
+    "hs_host_malloc": [C.POINTER(_P), C.c_size_t],
+    "hs_host_free": [_P],
+    "hs_device_memory": [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "hs_memcpy_htod": [_P, _P, _P, C.c_size_t],
     "hs_memcpy_dtoh": [_P, _P, _P, C.c_size_t],
     "hs_memcpy_dtod": [_P, _P, _P, C.c_size_t],

@@ -1,0 +1,19 @@
+// Text formats of the reference (src/utility/MatrixParser.cpp:11-274):
+// "# N" header, then N lower-triangle rows of ';'-separated values; the
+// right-hand side as "# N" and one ';'-separated row.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "hs/Queue.hpp"
+#include "hs/RightHandSide.hpp"
+#include "hs/SymmetricMatrix.hpp"
+
+class MatrixParser {
+public:
+    static SymmetricMatrix parseSymmetricMatrix(std::string& path, hs::Queue& queue);
+    static RightHandSide parseRightHandSide(std::string& path, hs::Queue& queue);
+    static std::vector<conf::fp_type> getRowValuesFromString(const std::string& rowString);
+    static void writeFullMatrix(const std::string& path, const SymmetricMatrix& matrix);
+};

@@ -54,6 +54,13 @@ int hs_device_count(int* count);
 /* Replaces sycl::malloc_device / free / queue.memcpy (Cholesky.cpp:24-32, CG.cpp:204-229). */
 int hs_malloc(void** ptr, size_t bytes);
 int hs_free(void* ptr);
+/* Pinned host memory; replaces sycl::usm_allocator<.., usm::alloc::host>
+ * (SymmetricMatrix.hpp:53, RightHandSide.hpp:29). */
+int hs_host_malloc(void** ptr, size_t bytes);
+int hs_host_free(void* ptr);
+/* Total / free device memory of the handle's device (the reference checks
+ * global_mem_size before allocating, CG.cpp:170-202). */
+int hs_device_memory(hs_handle_t handle, size_t* free_bytes, size_t* total_bytes);
 int hs_memcpy_htod(hs_handle_t handle, void* dst, const void* src, size_t bytes);
 int hs_memcpy_dtoh(hs_handle_t handle, void* dst, const void* src, size_t bytes);
 int hs_memcpy_dtod(hs_handle_t handle, void* dst, const void* src, size_t bytes);

@@ -1,0 +1,94 @@
+"""The C++ host API end to end on the GPU through the CLI driver (the
+reference's main.cpp flow): Cholesky + triangular solves on the reference's
+fixture, CG and Cholesky on GP inputs in the reference's input-file format,
+compared with the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from hs_testlib import GOLDEN_DIR, ROOT, dptr, fixture_matrix, fixture_vector, packed_len
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "heterogeneous-solvers_amd", "lib", "hs_driver")
+MATRIX = os.path.join(GOLDEN_DIR, "data", "testMatrixSymmetric20x20.txt")
+VECTOR = os.path.join(GOLDEN_DIR, "data", "testVector_20.txt")
+
+
+def drive(tmp_path, *args):
+    print("hs_driver", *args, flush=True)
+    r = subprocess.run([DRIVER] + [str(a) for a in args] + ["--output", str(tmp_path / "out")], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    return r
+
+
+def result(r, key):
+    m = re.search(r"%s=([-+0-9.eE]+)" % key, r.stdout)
+    return float(m.group(1))
+
+
+@pytest.mark.parametrize("bsz", [4, 6, 20])
+def test_driver_cholesky_on_reference_fixture(tmp_path, bsz):
+    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", bsz, "--init_gpu_perc", 1,
+              "--algorithm", "cholesky", "--write_result", "true", "--check_result", "true")
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(tmp_path / "x_result.txt")[:20]
+    ref = np.linalg.solve(fixture_matrix(), fixture_vector())
+    assert np.allclose(x, ref, rtol=1e-10, atol=1e-12)
+    assert result(r, "relative_residual") < 1e-12
+    assert os.path.exists(tmp_path / "out")
+
+
+def _gp_files(tmp_path, N, seed=7):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.standard_normal(N)) * 0.05
+    y = np.sin(np.arange(N) * 0.01) + 0.1 * rng.standard_normal(N)
+    np.savetxt(tmp_path / "gp_in.txt", t, fmt="%.17g")
+    np.savetxt(tmp_path / "gp_out.txt", y, fmt="%.17g")
+    return t, y
+
+
+def test_driver_cg_on_gp_input_files(tmp_path, oracle):
+    N, b = 1500, 64
+    t, y = _gp_files(tmp_path, N)
+    r = drive(tmp_path, "--gp_input", tmp_path / "gp_in.txt", "--gp_output", tmp_path / "gp_out.txt", "--size", N,
+              "--matrix_bsz", b, "--init_gpu_perc", 1, "--algorithm", "cg", "--eps", 1e-10, "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    A = np.zeros(packed_len(N, b))
+    oracle.or_rbf_window(dptr(t), N, b, dptr(A))
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    xref, itref, _ = oracle.cg(A, rhs, N, b, 1e-10)
+    x = np.loadtxt(tmp_path / "x_result.txt")
+    it = int(result(r, "iterations"))
+    print("iterations driver %d oracle %d" % (it, itref))
+    assert abs(it - itref) <= max(1, itref // 20)
+    assert np.linalg.norm(x - xref) <= 1e-6 * np.linalg.norm(xref)
+
+
+def test_driver_cholesky_on_gp_input_files(tmp_path, oracle):
+    N, b = 2048, 128
+    t, y = _gp_files(tmp_path, N, seed=8)
+    r = drive(tmp_path, "--gp_input", tmp_path / "gp_in.txt", "--gp_output", tmp_path / "gp_out.txt", "--size", N,
+              "--matrix_bsz", b, "--init_gpu_perc", 1, "--algorithm", "cholesky", "--write_result", "true",
+              "--check_result", "true")
+    assert r.returncode == 0, r.stderr
+    A = np.zeros(packed_len(N, b))
+    oracle.or_rbf_window(dptr(t), N, b, dptr(A))
+    rhs = np.zeros(N)
+    rhs[:N] = y
+    xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
+    x = np.loadtxt(tmp_path / "x_result.txt")
+    assert np.linalg.norm(x - xref) <= 1e-9 * np.linalg.norm(xref)
+    assert result(r, "relative_residual") < 1e-12
+
+
+def test_driver_rejects_partial_gpu_share(tmp_path):
+    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", 4, "--init_gpu_perc", 0.5,
+              "--algorithm", "cg")
+    assert r.returncode == 1
+    assert "share 1.0" in r.stderr
