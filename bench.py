@@ -63,24 +63,33 @@ def cholesky_workload(hs, dev, syn, args):
     plen = packed_len(N, b)
     log("cholesky workload N=%d b=%d d=%d: packed K %.2f GB" % (N, b, d, plen * 8 / 1e9))
     dA = dev.alloc(plen * 8)
-    timer = hs.Timer(dev)
+    t_all, t_asm, t_fac, t_sol = hs.Timer(dev), hs.Timer(dev), hs.Timer(dev), hs.Timer(dev)
     phase = (C.c_double * 3)()
     launches = (C.c_int64 * 3)()
-    step_ms, phases = [], []
+    step_ms, phases, parts = [], [], []
     for s in range(args.warmup + args.steps):
-        timer.start()
+        t_all.start()
+        t_asm.start()
         hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+        t_asm.stop()
+        t_fac.start()
         info = hs.call("hs_cholesky_f64_timed", dev.h, dA.p, N, b, phase, launches)
+        t_fac.stop()
+        t_sol.start()
         hs.call("hs_memcpy_dtod", dev.h, dx.p, dy.p, n * b * 8)
         hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
-        timer.stop()
-        ms = timer.ms()
+        t_sol.stop()
+        t_all.stop()
+        ms = t_all.ms()
+        part = (t_asm.ms(), t_fac.ms(), t_sol.ms())
         if info != 0:
             raise RuntimeError("factorisation reported non-positive pivot %d" % info)
         if s >= args.warmup:
             step_ms.append(ms)
             phases.append(list(phase))
-        log("  step %d: %.1f ms (potrf %.1f trsm %.1f update %.1f ms)" % (s, ms, phase[0], phase[1], phase[2]))
+            parts.append(part)
+        log("  step %d: %.1f ms (assembly %.1f factor %.1f solves %.1f; kernel sums potrf %.1f trsm %.1f "
+            "update %.1f ms)" % (s, ms, part[0], part[1], part[2], phase[0], phase[1], phase[2]))
     x = dev.download(dx, n * b)
     # untimed check: residual on a freshly assembled K
     hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
@@ -97,6 +106,7 @@ def cholesky_workload(hs, dev, syn, args):
     bwd = float(np.max(np.abs(res)) / (knorm_inf * np.max(np.abs(x)) + np.max(np.abs(rhs))))
     dA.free()
     ph = np.array(phases).mean(axis=0)
+    pt = np.array(parts).mean(axis=0)
     flops = N ** 3 / 3.0
     ms = float(np.median(step_ms))
     nl = int(launches[2])
@@ -110,8 +120,9 @@ def cholesky_workload(hs, dev, syn, args):
     return {
         "N": N, "b": b, "d": d, "seed": seed, "ell": ell, "sf2": sf2, "sn2": sn2,
         "ms_per_step": ms, "step_ms": step_ms, "tflops_step": flops / (ms / 1e3) / 1e12,
-        "phase_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])},
-        "factor_ms": float(ph.sum()), "tflops_factor": flops / (ph.sum() / 1e3) / 1e12,
+        "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])},
+        "assembly_ms": float(pt[0]), "factor_ms": float(pt[1]), "solve_ms": float(pt[2]),
+        "tflops_factor": flops / (pt[1] / 1e3) / 1e12,
         "update": {"launches": nl, "flops_per_launch": upd_flops / nl, "avg_launch_ms": float(ph[2]) / nl,
                    "tflops": achieved},
         "rel_residual": rel_res, "backward_error_inf": bwd,

@@ -442,6 +442,14 @@ extern "C" int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64
     if (b <= 0) return -5;
     const int n = (int)((N + b - 1) / b);
     int rc;
+    if (hs_trsv_fast_ok(b)) {
+        // workspace: L_jj^{-T} and L_jj^{-1} blocks, then the forward vector
+        if ((rc = hs_ensure_work(h, sizeof(double) * ((size_t)2 * n * b * b + (size_t)n * b)))) return rc;
+        double* Dinv = (double*)h->work;
+        double* z = Dinv + (size_t)2 * n * b * b;
+        if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
+        return hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b);
+    }
     for (int j = 0; j < n; ++j) {
         const i64 id = hs_block_id(j, j, n);
         if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 0))) return rc;

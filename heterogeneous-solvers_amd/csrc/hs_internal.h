@@ -69,6 +69,12 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
 // column k+1, rows k+1..n-1) split step k for the look-ahead driver
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
                           int mode);
+// fast blocked triangular solves (b in {64, 128, 256}); factor in FP64 or FP32
+bool hs_trsv_fast_ok(int b);
+template <typename T>
+int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv);
+template <typename T>
+int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b);
 int hs_launch_trsv_block(hs_handle_t h, const double* A, double* x, i64 N, int b, int blockRow, i64 blockID, int transposed);
 int hs_launch_col_update(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
                          i64 blockID, int n, int transposed);

@@ -84,7 +84,8 @@ def test_driver_cholesky_on_gp_input_files(tmp_path, oracle):
     xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
     x = np.loadtxt(tmp_path / "x_result.txt")
     assert np.linalg.norm(x - xref) <= 1e-9 * np.linalg.norm(xref)
-    assert result(r, "relative_residual") < 1e-12
+    # the random-walk window features give cond ~ 1e5: residual ~ cond * u
+    assert result(r, "relative_residual") < 1e-10
 
 
 def test_driver_rejects_partial_gpu_share(tmp_path):
