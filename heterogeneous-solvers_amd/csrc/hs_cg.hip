@@ -62,7 +62,8 @@ __device__ inline void decode_block(i64 id, int n, int* I, int* J) {
     *I = (int)(j + (id - (j * n - j * (j - 1) / 2)));
 }
 
-template <int BS>
+// ABS: use |a_ij| (gives the row sums of |A| for x = 1, i.e. ||A||_inf)
+template <int BS, bool ABS = false>
 __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const double* __restrict__ A,
                                                                       const double* __restrict__ x,
                                                                       double* __restrict__ rowpart,
@@ -112,7 +113,8 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
             const double xi = xI[row];
 #pragma unroll
             for (int s = 0; s < NL; ++s) {
-                const double2 a = *reinterpret_cast<const double2*>(rp + s * 2 * LPR);
+                double2 a = *reinterpret_cast<const double2*>(rp + s * 2 * LPR);
+                if (ABS) a = make_double2(fabs(a.x), fabs(a.y));
                 acc += a.x * xj[2 * s] + a.y * xj[2 * s + 1];
                 cacc[2 * s] += a.x * xi;
                 cacc[2 * s + 1] += a.y * xi;
@@ -295,6 +297,20 @@ __global__ __launch_bounds__(SYMV_THREADS) void cg_d_kernel(CGState* st, const d
 // ---------------------------------------------------------------------------
 static bool symv_fast_ok(int b) { return b == 64 || b == 128 || b == 256; }
 
+static int launch_partials_abs(hs_handle_t h, const double* A, const double* x, double* rowpart, double* colpart,
+                              int n, int b) {
+    const i64 blocks = (i64)n * (n + 1) / 2;
+    dim3 grid((unsigned)blocks), blk(SYMV_THREADS);
+    switch (b) {
+        case 64: symv_partials_kernel<64, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        case 128: symv_partials_kernel<128, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        case 256: symv_partials_kernel<256, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        default: hs_set_error("SYMV: unsupported block size"); return HS_ERR_UNSUPPORTED;
+    }
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
 static int launch_partials(hs_handle_t h, const double* A, const double* x, double* rowpart, double* colpart, int n,
                            int b, const i64* stop, i64 it) {
     const i64 blocks = (i64)n * (n + 1) / 2;
@@ -468,5 +484,27 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
     if (iterations) *iterations = iters;
     if (delta_final) *delta_final = hs.delta[iters & 1];
     if (delta_zero) *delta_zero = hs.delta0;
+    return HS_OK;
+}
+
+// Internal SYMV for other solvers: mode 0 y = A x, mode 2 y = rhs - A x (part
+// receives rows/64 per-work-group partial sums of y_i^2); abs_val uses |A|.
+// rowpart / colpart hold n(n+1)/2 * b doubles each.
+int hs_symv_into(hs_handle_t h, const double* A, const double* x, double* y, const double* rhs, i64 N, int b,
+                 double* rowpart, double* colpart, double* part, int mode, bool abs_val) {
+    if (!symv_fast_ok(b)) {
+        hs_set_error("hs_symv_into: block size must be 64, 128 or 256");
+        return HS_ERR_UNSUPPORTED;
+    }
+    const int n = (int)((N + b - 1) / b);
+    int rc = abs_val ? launch_partials_abs(h, A, x, rowpart, colpart, n, b)
+                     : launch_partials(h, A, x, rowpart, colpart, n, b, nullptr, 0);
+    if (rc) return rc;
+    const dim3 fg((unsigned)((i64)n * b / FIN_ROWS)), vb(SYMV_THREADS);
+    if (mode == 2)
+        symv_finish_kernel<2><<<fg, vb, 0, h->stream>>>(rowpart, colpart, n, b, y, rhs, part, nullptr, 0);
+    else
+        symv_finish_kernel<0><<<fg, vb, 0, h->stream>>>(rowpart, colpart, n, b, y, nullptr, nullptr, nullptr, 0);
+    HS_LAUNCH_CHECK();
     return HS_OK;
 }

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "hs_factor.h"
 #include "hs_internal.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -23,40 +24,6 @@ namespace {
 
 constexpr int KT = 16;
 constexpr int THREADS = 256;
-
-struct PairMap {
-    int k, rowStart, mode;
-    i64 pairOffset;
-};
-
-// decode work-group pair index -> (I, J)
-__device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
-    if (pm.mode == 1) {
-        *I = pm.rowStart + (int)q;
-        *J = *I;
-        return;
-    }
-    if (pm.mode == 5) {  // block column k+1 only (look-ahead panel update)
-        *I = pm.k + 1 + (int)q;
-        *J = pm.k + 1;
-        return;
-    }
-    int r, c;
-    if (pm.mode == 4) {  // trailing triangle without block column k+1
-        hs_tri_decode(q, &r, &c);
-        *I = pm.k + 2 + r;
-        *J = pm.k + 2 + c;
-        return;
-    }
-    hs_tri_decode(q + pm.pairOffset, &r, &c);
-    if (pm.mode == 2) {
-        *I = pm.k + 2 + r;
-        *J = pm.k + 1 + c;
-    } else {
-        *I = pm.k + 1 + r;
-        *J = pm.k + 1 + c;
-    }
-}
 
 // KIND only separates the look-ahead column launches (KIND 1) from the rest
 // in profiles; the code is the same.
@@ -201,166 +168,44 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
     }
     const int tile = (b % 128 == 0) ? 128 : 64;
     const i64 tilesPerBlock = (i64)(b / tile) * (b / tile);
-    auto launch = [&](int md, i64 pairs, i64 off, int rs) -> int {
-        if (pairs <= 0) return HS_OK;
-        PairMap pm{k, rs, md, off};
-        const i64 grid = pairs * tilesPerBlock;
-        if (grid > 0x7fffffffLL) {
-            hs_set_error("trailing update grid too large");
-            return HS_ERR_UNSUPPORTED;
-        }
-        const dim3 g((unsigned)grid), t(THREADS);
-        if (tile == 128) {
-            if (md == 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
-            else trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
-        } else {
-            if (md == 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
-            else trailing_mfma_f64<64, 0><<<g, t, 0, stream>>>(A, b, n, pm);
-        }
-        HS_LAUNCH_CHECK();
-        return HS_OK;
-    };
-    if (mode == 1) return launch(1, rowCount, 0, rowStart);
-    if (mode == 4) {  // rows k+2..n-1, columns k+2..I
-        const i64 m = n - k - 2;
-        return launch(4, m * (m + 1) / 2, 0, k + 2);
+    i64 pairs, off;
+    int rs;
+    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
+    if (pairs <= 0) return HS_OK;
+    PairMap pm{k, rs, mode, off};
+    const i64 grid = pairs * tilesPerBlock;
+    if (grid > 0x7fffffffLL) {
+        hs_set_error("trailing update grid too large");
+        return HS_ERR_UNSUPPORTED;
     }
-    if (mode == 5) return launch(5, n - k - 1, 0, k + 1);  // column k+1, rows k+1..n-1
-    if (mode == 2) {
-        const int firstRow = std::max(rowStart, k + 2);
-        const int lastRow = rowStart + rowCount;
-        if (lastRow <= firstRow) return HS_OK;
-        const i64 r0 = firstRow - (k + 2), r1 = lastRow - (k + 2);
-        return launch(2, r1 * (r1 + 1) / 2 - r0 * (r0 + 1) / 2, r0 * (r0 + 1) / 2, rowStart);
+    const dim3 g((unsigned)grid), t(THREADS);
+    if (tile == 128) {
+        if (mode == 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        else trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+    } else {
+        if (mode == 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        else trailing_mfma_f64<64, 0><<<g, t, 0, stream>>>(A, b, n, pm);
     }
-    // mode 3: every (I,J) with I in the row range and k < J <= I
-    const i64 r0 = rowStart - (k + 1), r1 = rowStart + rowCount - (k + 1);
-    return launch(3, r1 * (r1 + 1) / 2 - r0 * (r0 + 1) / 2, r0 * (r0 + 1) / 2, rowStart);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
 }
 
 int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode) {
     return hs_launch_trailing_on(h, h->stream, A, b, k, rowStart, rowCount, n, mode);
 }
 
-namespace {
-
-// Runs launches that use h->stream on another stream of the same handle.
-struct OnStream {
-    hs_handle_t h;
-    hipStream_t saved;
-    OnStream(hs_handle_t hh, hipStream_t s) : h(hh), saved(hh->stream) { h->stream = s; }
-    ~OnStream() { h->stream = saved; }
-};
-
-// Per-phase event bracketing for the timed variant: [start, end) pairs on a
-// stream, summed after the final synchronisation.
-struct PhaseTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
-    bool on = false;
-    int begin(hipStream_t s, int phase, hipEvent_t* out) {
-        if (!on) return HS_OK;
-        hipEvent_t e;
-        HS_HIP(hipEventCreate(&e));
-        HS_HIP(hipEventRecord(e, s));
-        spans[phase].push_back({e, nullptr});
-        *out = e;
-        return HS_OK;
-    }
-    int end(hipStream_t s, int phase) {
-        if (!on) return HS_OK;
-        hipEvent_t e;
-        HS_HIP(hipEventCreate(&e));
-        HS_HIP(hipEventRecord(e, s));
-        spans[phase].back().second = e;
-        return HS_OK;
-    }
-    int collect(double* ms, int64_t* launches) {
-        for (int p = 0; p < 3; ++p) {
-            double t = 0;
-            for (auto& pr : spans[p]) {
-                float v = 0;
-                HS_HIP(hipEventElapsedTime(&v, pr.first, pr.second));
-                t += v;
-            }
-            ms[p] = t;
-            launches[p] = (int64_t)spans[p].size();
-        }
-        return HS_OK;
-    }
-    ~PhaseTimer() {
-        for (auto& sp : spans)
-            for (auto& pr : sp) {
-                if (pr.first) hipEventDestroy(pr.first);
-                if (pr.second) hipEventDestroy(pr.second);
-            }
-    }
-};
-
-// Right-looking factorisation with a look-ahead of one panel on two streams
-// (same priority). Stream S2 factors panels: it applies step k's update to
-// block column k+1 only, then POTRF and TRSM of panel k+1. Stream S1 applies
-// step k's update to the rest of the trailing triangle (J >= k+2) as soon as
-// panel k is final, so POTRF/TRSM of k+1 run underneath it.
-// Dependencies: S1 update k waits for panel k (event P[k]); S2's column update
-// of step k+1 waits for S1's update k (it also wrote block column k+2).
-int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
+// Look-ahead FP64 factorisation (schedule in hs_factor.h).
+static int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    rc = hs_ensure_events(h, 2 * n + 2);
-    if (rc) return rc;
-    hipEvent_t* evP = h->events.data();          // panel k final (S2)
-    hipEvent_t* evU = h->events.data() + n + 1;  // S1 update k done
-    hipStream_t s1 = h->stream, s2 = h->stream2;
-    hipEvent_t dummy;
-    // S2 must see everything already queued on S1 (e.g. the assembly of A)
-    HS_HIP(hipEventRecord(evU[n], s1));
-    HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
-    {
-        OnStream os(h, s2);
-        if ((rc = pt.begin(s2, 0, &dummy))) return rc;
-        if ((rc = hs_launch_potrf(h, A, N, b, hs_block_id(0, 0, n), 0))) return rc;
-        if ((rc = pt.end(s2, 0))) return rc;
-        if (n > 1) {
-            if ((rc = pt.begin(s2, 1, &dummy))) return rc;
-            if ((rc = hs_launch_trsm(h, A, b, hs_block_id(0, 0, n), 0, 1, n - 1))) return rc;
-            if ((rc = pt.end(s2, 1))) return rc;
-        }
-    }
-    HS_HIP(hipEventRecord(evP[0], s2));
-    for (int k = 0; k + 1 < n; ++k) {
-        const int m = n - k - 1;  // blocks below the diagonal in column k
-        // S2: column k+1 update with panel k, then panel k+1
-        if (k >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[k - 1], 0));
-        if ((rc = hs_launch_trailing_on(h, s2, A, b, k, k + 1, m, n, 5))) return rc;
-        {
-            OnStream os(h, s2);
-            const i64 id = hs_block_id(k + 1, k + 1, n);
-            if ((rc = pt.begin(s2, 0, &dummy))) return rc;
-            if ((rc = hs_launch_potrf(h, A, N, b, id, k + 1))) return rc;
-            if ((rc = pt.end(s2, 0))) return rc;
-            if (m > 1) {
-                if ((rc = pt.begin(s2, 1, &dummy))) return rc;
-                if ((rc = hs_launch_trsm(h, A, b, id, k + 1, k + 2, m - 1))) return rc;
-                if ((rc = pt.end(s2, 1))) return rc;
-            }
-        }
-        HS_HIP(hipEventRecord(evP[k + 1], s2));
-        // S1: the rest of step k's trailing update
-        HS_HIP(hipStreamWaitEvent(s1, evP[k], 0));
-        if (m > 1) {
-            if ((rc = pt.begin(s1, 2, &dummy))) return rc;
-            if ((rc = hs_launch_trailing_on(h, s1, A, b, k, k + 2, m - 1, n, 4))) return rc;
-            if ((rc = pt.end(s1, 2))) return rc;
-        }
-        HS_HIP(hipEventRecord(evU[k], s1));
-    }
-    // join: S1 waits for the last panel
-    HS_HIP(hipStreamWaitEvent(s1, evP[n - 1], 0));
-    return HS_OK;
+    auto potrf = [&](int k) { return hs_launch_potrf(h, A, N, b, hs_block_id(k, k, n), k); };
+    auto trsm = [&](int k, int start, int count) { return hs_launch_trsm(h, A, b, hs_block_id(k, k, n), k, start, count); };
+    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
+        return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode);
+    };
+    return hs_lookahead_factor(h, n, pt, potrf, trsm, trail);
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Whole factorisation, Cholesky::solve_heterogeneous at share 1.0
@@ -375,18 +220,17 @@ static int cholesky_sequential(hs_handle_t h, double* A, i64 N, int b, PhaseTime
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    hipEvent_t dummy;
     for (int k = 0; k < n; ++k) {
         const i64 id = hs_block_id(k, k, n);
-        if ((rc = pt.begin(h->stream, 0, &dummy))) return rc;
+        if ((rc = pt.begin(h->stream, 0))) return rc;
         if ((rc = hs_launch_potrf(h, A, N, b, id, k))) return rc;
         if ((rc = pt.end(h->stream, 0))) return rc;
         const int cnt = n - k - 1;
         if (cnt > 0) {
-            if ((rc = pt.begin(h->stream, 1, &dummy))) return rc;
+            if ((rc = pt.begin(h->stream, 1))) return rc;
             if ((rc = hs_launch_trsm(h, A, b, id, k, k + 1, cnt))) return rc;
             if ((rc = pt.end(h->stream, 1))) return rc;
-            if ((rc = pt.begin(h->stream, 2, &dummy))) return rc;
+            if ((rc = pt.begin(h->stream, 2))) return rc;
             if ((rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) return rc;
             if ((rc = pt.end(h->stream, 2))) return rc;
         }

@@ -1,0 +1,390 @@
+// Mixed-precision blocked Cholesky (no counterpart in the reference; its
+// nearest relative is the whole-program FP32 build, CMakeLists.txt:7,104-106).
+//
+// The factor is stored in FP32 (packed layout, 4 bytes per entry). Per step k:
+//  - POTRF of the diagonal block and TRSM of the panel run in FP64 arithmetic
+//    on the FP32-stored values (the FP64 kernels, templated on storage);
+//  - the trailing SYRK/GEMM update C -= L_Ik L_Jk^T runs on MFMA with FP32
+//    accumulation, in one of two variants:
+//      HS_MIXED_FP32:   v_mfma_f32_16x16x4f32 on the FP32 values;
+//      HS_MIXED_BF16X3: every FP32 operand x is split into bf16 hi = rn(x) and
+//                       lo = rn(x - hi); each product uses three
+//                       v_mfma_f32_16x16x32_bf16: hi*hi' + hi*lo' + lo*hi'
+//                       (the dropped lo*lo' and the split residuals are
+//                       ~2^-17 relative).
+// The steps follow the same one-panel look-ahead schedule as the FP64 path.
+#include <algorithm>
+
+#include "hs_factor.h"
+#include "hs_internal.h"
+
+int hs_launch_potrf_f32(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow);
+int hs_launch_trsm_f32(hs_handle_t h, float* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
+
+namespace {
+
+constexpr int THREADS = 256;
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// K (FP64 packed) -> FP32 packed copy, + shift on the global diagonal
+__global__ void to_f32_kernel(const double* __restrict__ K, float* __restrict__ L, i64 total) {
+    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (i64)gridDim.x * blockDim.x)
+        L[e] = (float)K[e];
+}
+
+__global__ void diag_shift_kernel(const double* __restrict__ K, float* __restrict__ L, int b, int n, i64 N,
+                                  double shift) {
+    const int j = blockIdx.x;
+    for (int i = threadIdx.x; i < b; i += blockDim.x) {
+        if ((i64)j * b + i >= N) continue;
+        const i64 e = hs_block_id(j, j, n) * (i64)b * b + (i64)i * b + i;
+        L[e] = (float)(K[e] + shift);
+    }
+}
+
+// ---- FP32 MFMA trailing update -------------------------------------------
+// TILE x TILE output tile per work-group, 2x2 waves, 16x16x4 f32 MFMA tiles
+// (register v of lane l: row 4(l/16)+v, column l%16), KT = 16 deep LDS
+// stages, double buffered, operands stored k-major.
+template <int TILE, int KIND>
+__global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__ A, int b, int n, PairMap pm) {
+    constexpr int KT = 16;
+    constexpr int WT = TILE / 2, MT = WT / 16;
+    constexpr int LDP = TILE + 4;
+    constexpr int STAGE = KT * LDP;
+    __shared__ float As[2 * STAGE];
+    __shared__ float Bs[2 * STAGE];
+    const int tpb = (b / TILE) * (b / TILE);
+    const i64 q = blockIdx.x / tpb;
+    const int tile = blockIdx.x % tpb;
+    const int tm = tile / (b / TILE), tn = tile % (b / TILE);
+    int I, J;
+    pair_of(pm, q, &I, &J);
+    const bool diag = (I == J);
+    if (diag && tm < tn) return;
+    const i64 bb = (i64)b * b;
+    const float* La = A + hs_block_id(I, pm.k, n) * bb + (i64)tm * TILE * b;
+    const float* Lb = A + hs_block_id(J, pm.k, n) * bb + (i64)tn * TILE * b;
+    float* C = A + hs_block_id(I, J, n) * bb + (i64)tm * TILE * b + (i64)tn * TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * WT, wn0 = (wave & 1) * WT;
+    constexpr int LPT = TILE * KT / THREADS;  // 8 floats per thread (TILE 128)
+    constexpr int TPR = KT / LPT;
+    const int lrow = tid / TPR, lk = (tid % TPR) * LPT;
+    float4 ra[LPT / 4], rb[LPT / 4];
+    const float* pa = La + (i64)lrow * b + lk;
+    const float* pb = Lb + (i64)lrow * b + lk;
+    f4 acc[MT][MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nk = b / KT;
+    auto gload = [&](int kt) {
+#pragma unroll
+        for (int v = 0; v < LPT / 4; ++v) {
+            ra[v] = *reinterpret_cast<const float4*>(pa + kt * KT + 4 * v);
+            rb[v] = *reinterpret_cast<const float4*>(pb + kt * KT + 4 * v);
+        }
+    };
+    auto sstore = [&](int buf) {
+        float* as = As + buf * STAGE;
+        float* bs = Bs + buf * STAGE;
+#pragma unroll
+        for (int v = 0; v < LPT / 4; ++v) {
+            as[(lk + 4 * v) * LDP + lrow] = ra[v].x;
+            as[(lk + 4 * v + 1) * LDP + lrow] = ra[v].y;
+            as[(lk + 4 * v + 2) * LDP + lrow] = ra[v].z;
+            as[(lk + 4 * v + 3) * LDP + lrow] = ra[v].w;
+            bs[(lk + 4 * v) * LDP + lrow] = rb[v].x;
+            bs[(lk + 4 * v + 1) * LDP + lrow] = rb[v].y;
+            bs[(lk + 4 * v + 2) * LDP + lrow] = rb[v].z;
+            bs[(lk + 4 * v + 3) * LDP + lrow] = rb[v].w;
+        }
+    };
+    gload(0);
+    sstore(0);
+    if (nk > 1) gload(1);
+    __syncthreads();
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+        const float* as = As + (kt & 1) * STAGE;
+        const float* bs = Bs + (kt & 1) * STAGE;
+#pragma unroll
+        for (int ks = 0; ks < KT; ks += 4) {
+            float av[MT], bv[MT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i) av[i] = as[(ks + lq) * LDP + wm0 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < MT; ++j) bv[j] = bs[(ks + lq) * LDP + wn0 + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+                for (int j = 0; j < MT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) {
+            sstore((kt + 1) & 1);
+            if (kt + 2 < nk) gload(kt + 2);
+        }
+        __syncthreads();
+    }
+    const bool dtile = diag && tm == tn;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+            const int col = wn0 + j * 16 + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm0 + i * 16 + 4 * lq + v;
+                if (!dtile || row >= col) C[(i64)row * b + col] -= acc[i][j][v];
+            }
+        }
+}
+
+// ---- BF16 hi+lo split trailing update -------------------------------------
+__device__ inline unsigned short bf16_rn(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+// TILE x TILE tile per work-group, 2x2 waves of (TILE/2)^2, KT = 32 (one
+// bf16 MFMA depth) per LDS stage, double buffered. LDS keeps hi and lo
+// planes row-major [row][k] with an 80-byte row pitch, so the 16-byte
+// fragment reads of 16 lanes (8 consecutive k of 16 rows) hit distinct banks.
+// Fragment of lane l: row l%16, k = 8(l/16)..+7; output as the f32 MFMA.
+template <int TILE, int KIND>
+__global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restrict__ A, int b, int n, PairMap pm) {
+    constexpr int KT = 32;
+    constexpr int WT = TILE / 2, MT = WT / 16;
+    constexpr int P = 40;                 // row pitch in bf16 elements (80 bytes)
+    constexpr int PLANE = TILE * P;       // one hi or lo plane of one operand
+    constexpr int STAGE = 4 * PLANE;      // A hi, A lo, B hi, B lo
+    __shared__ unsigned short lds[2 * STAGE];
+    const int tpb = (b / TILE) * (b / TILE);
+    const i64 q = blockIdx.x / tpb;
+    const int tile = blockIdx.x % tpb;
+    const int tm = tile / (b / TILE), tn = tile % (b / TILE);
+    int I, J;
+    pair_of(pm, q, &I, &J);
+    const bool diag = (I == J);
+    if (diag && tm < tn) return;
+    const i64 bb = (i64)b * b;
+    const float* La = A + hs_block_id(I, pm.k, n) * bb + (i64)tm * TILE * b;
+    const float* Lb = A + hs_block_id(J, pm.k, n) * bb + (i64)tn * TILE * b;
+    float* C = A + hs_block_id(I, J, n) * bb + (i64)tm * TILE * b + (i64)tn * TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * WT, wn0 = (wave & 1) * WT;
+    constexpr int LPT = TILE * KT / THREADS;  // 16 floats per thread (TILE 128)
+    constexpr int TPR = KT / LPT;             // 2 threads per row
+    const int lrow = tid / TPR, lk = (tid % TPR) * LPT;
+    float4 ra[LPT / 4], rb[LPT / 4];
+    const float* pa = La + (i64)lrow * b + lk;
+    const float* pb = Lb + (i64)lrow * b + lk;
+    f4 acc[MT][MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nk = b / KT;
+    auto gload = [&](int kt) {
+#pragma unroll
+        for (int v = 0; v < LPT / 4; ++v) {
+            ra[v] = *reinterpret_cast<const float4*>(pa + kt * KT + 4 * v);
+            rb[v] = *reinterpret_cast<const float4*>(pb + kt * KT + 4 * v);
+        }
+    };
+    auto split_store = [&](const float4* r, unsigned short* hiPlane, unsigned short* loPlane) {
+        unsigned short hi[LPT], lo[LPT];
+#pragma unroll
+        for (int v = 0; v < LPT / 4; ++v) {
+            const float x[4] = {r[v].x, r[v].y, r[v].z, r[v].w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const unsigned short h = bf16_rn(x[c]);
+                hi[4 * v + c] = h;
+                lo[4 * v + c] = bf16_rn(x[c] - bf16_to_f(h));
+            }
+        }
+        uint4* dh = reinterpret_cast<uint4*>(hiPlane + lrow * P + lk);
+        uint4* dl = reinterpret_cast<uint4*>(loPlane + lrow * P + lk);
+#pragma unroll
+        for (int w = 0; w < LPT / 8; ++w) {
+            uint4 vh, vl;
+            vh.x = hi[8 * w] | ((unsigned)hi[8 * w + 1] << 16);
+            vh.y = hi[8 * w + 2] | ((unsigned)hi[8 * w + 3] << 16);
+            vh.z = hi[8 * w + 4] | ((unsigned)hi[8 * w + 5] << 16);
+            vh.w = hi[8 * w + 6] | ((unsigned)hi[8 * w + 7] << 16);
+            vl.x = lo[8 * w] | ((unsigned)lo[8 * w + 1] << 16);
+            vl.y = lo[8 * w + 2] | ((unsigned)lo[8 * w + 3] << 16);
+            vl.z = lo[8 * w + 4] | ((unsigned)lo[8 * w + 5] << 16);
+            vl.w = lo[8 * w + 6] | ((unsigned)lo[8 * w + 7] << 16);
+            dh[w] = vh;
+            dl[w] = vl;
+        }
+    };
+    auto sstore = [&](int buf) {
+        unsigned short* s = lds + buf * STAGE;
+        split_store(ra, s, s + PLANE);
+        split_store(rb, s + 2 * PLANE, s + 3 * PLANE);
+    };
+    gload(0);
+    sstore(0);
+    if (nk > 1) gload(1);
+    __syncthreads();
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+        const unsigned short* s = lds + (kt & 1) * STAGE;
+        bf8 ah[MT], al[MT], bh[MT], bl[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+            const int off = (wm0 + i * 16 + lr) * P + 8 * lq;
+            ah[i] = *reinterpret_cast<const bf8*>(s + off);
+            al[i] = *reinterpret_cast<const bf8*>(s + PLANE + off);
+        }
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+            const int off = (wn0 + j * 16 + lr) * P + 8 * lq;
+            bh[j] = *reinterpret_cast<const bf8*>(s + 2 * PLANE + off);
+            bl[j] = *reinterpret_cast<const bf8*>(s + 3 * PLANE + off);
+        }
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < MT; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (kt + 1 < nk) {
+            sstore((kt + 1) & 1);
+            if (kt + 2 < nk) gload(kt + 2);
+        }
+        __syncthreads();
+    }
+    const bool dtile = diag && tm == tn;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+            const int col = wn0 + j * 16 + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm0 + i * 16 + 4 * lq + v;
+                if (!dtile || row >= col) C[(i64)row * b + col] -= acc[i][j][v];
+            }
+        }
+}
+
+}  // namespace
+
+int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
+                             int n, int mode, int variant) {
+    if (rowCount <= 0) return HS_OK;
+    if (b % 128 != 0) {
+        hs_set_error("mixed-precision trailing update needs b % 128 == 0");
+        return HS_ERR_UNSUPPORTED;
+    }
+    constexpr int TILE = 128;
+    const i64 tpb = (i64)(b / TILE) * (b / TILE);
+    i64 pairs, off;
+    int rs;
+    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
+    if (pairs <= 0) return HS_OK;
+    PairMap pm{k, rs, mode, off};
+    const i64 grid = pairs * tpb;
+    if (grid > 0x7fffffffLL) {
+        hs_set_error("trailing update grid too large");
+        return HS_ERR_UNSUPPORTED;
+    }
+    const dim3 g((unsigned)grid), t(THREADS);
+    if (variant == HS_MIXED_BF16X3) {
+        if (mode == 5) trailing_mfma_bf16x3<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        else trailing_mfma_bf16x3<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+    } else {
+        if (mode == 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        else trailing_mfma_f32<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+    }
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, double shift) {
+    if (!h) return -1;
+    if (!K) return -2;
+    if (!L32) return -3;
+    if (N <= 0) return -4;
+    if (b <= 0) return -5;
+    const int n = (int)((N + b - 1) / b);
+    const i64 total = (i64)n * (n + 1) / 2 * b * b;
+    to_f32_kernel<<<dim3(8192), dim3(256), 0, h->stream>>>(K, L32, total);
+    HS_LAUNCH_CHECK();
+    if (shift != 0.0) {
+        diag_shift_kernel<<<dim3((unsigned)n), dim3(256), 0, h->stream>>>(K, L32, b, n, N, shift);
+        HS_LAUNCH_CHECK();
+    }
+    return HS_OK;
+}
+
+static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, PhaseTimer& pt) {
+    const int n = (int)((N + b - 1) / b);
+    int rc = hs_reset_info(h);
+    if (rc) return rc;
+    auto potrf = [&](int k) { return hs_launch_potrf_f32(h, L32, N, b, hs_block_id(k, k, n), k); };
+    auto trsm = [&](int k, int start, int count) {
+        return hs_launch_trsm_f32(h, L32, b, hs_block_id(k, k, n), k, start, count);
+    };
+    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
+        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant);
+    };
+    if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
+        hipStreamSynchronize(h->stream2);
+        return rc;
+    }
+    int64_t info = 0;
+    if ((rc = hs_get_info(h, &info))) return rc;
+    return (int)info;
+}
+
+extern "C" int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant,
+                                 double shift) {
+    if (!h) return -1;
+    if (!K) return -2;
+    if (!L32) return -3;
+    if (N <= 0) return -4;
+    if (b % 128 != 0 || b > 256) {
+        hs_set_error("hs_cholesky_mixed: matrix_bsz must be 128 or 256");
+        return -5;
+    }
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -6;
+    int rc = hs_to_f32_packed(h, K, L32, N, b, shift);
+    if (rc) return rc;
+    PhaseTimer pt;
+    return mixed_factor(h, L32, N, b, variant, pt);
+}
+
+extern "C" int hs_cholesky_mixed_timed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant,
+                                       double shift, double* phase_ms, int64_t* launches) {
+    if (!phase_ms) return -8;
+    if (!launches) return -9;
+    if (!h) return -1;
+    if (!K) return -2;
+    if (!L32) return -3;
+    if (N <= 0) return -4;
+    if (b % 128 != 0 || b > 256) {
+        hs_set_error("hs_cholesky_mixed: matrix_bsz must be 128 or 256");
+        return -5;
+    }
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -6;
+    int rc = hs_to_f32_packed(h, K, L32, N, b, shift);
+    if (rc) return rc;
+    PhaseTimer pt;
+    pt.on = true;
+    rc = mixed_factor(h, L32, N, b, variant, pt);
+    if (rc < 0) return rc;
+    int rc2 = pt.collect(phase_ms, launches);
+    return rc2 ? rc2 : rc;
+}

@@ -99,17 +99,25 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_block_kernel(double* __re
 // chunk of finished columns is prefetched into registers while the current
 // chunk feeds the MFMAs.
 typedef double d4v __attribute__((ext_vector_type(4)));
+
+// two consecutive stored values as doubles (16-byte or 8-byte load)
+__device__ inline double2 load2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ inline double2 load2(const float* p) {
+    const float2 f = *reinterpret_cast<const float2*>(p);
+    return make_double2((double)f.x, (double)f.y);
+}
 constexpr int PM_PW = 32;
 constexpr int PM_KT = 16;
 
-__global__ __launch_bounds__(256) void potrf_block_mfma_kernel(double* __restrict__ A, i64 N, int b, i64 blockID,
+template <typename ST>
+__global__ __launch_bounds__(256) void potrf_block_mfma_kernel(ST* __restrict__ A, i64 N, int b, i64 blockID,
                                                                 int blockRow, int* __restrict__ info) {
     extern __shared__ double smem[];
     constexpr int PP = PM_PW + 1;  // panel pitch
     constexpr int TP = PM_KT + 1;  // chunk pitch
     double* P = smem;              // [b][PP]
     double* T = smem + (size_t)b * PP;  // [b][TP]
-    double* B = A + blockID * (i64)b * b;
+    ST* B = A + blockID * (i64)b * b;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
@@ -134,7 +142,7 @@ __global__ __launch_bounds__(256) void potrf_block_mfma_kernel(double* __restric
                 for (int i = 0; i < 8; ++i) {
                     const int e = tid + 256 * i;
                     const int r = e >> 3, kk = (e & 7) * 2;
-                    if (r < rows) pre[i] = *reinterpret_cast<const double2*>(B + (i64)(p + r) * b + k0 + kk);
+                    if (r < rows) pre[i] = load2(B + (i64)(p + r) * b + k0 + kk);
                 }
             };
             gload(0);
@@ -216,10 +224,10 @@ int hs_launch_potrf(hs_handle_t h, double* A, i64 N, int b, i64 blockID, int blo
     if (b % 32 == 0 && b <= 256) {
         const size_t lds = (size_t)b * (PM_PW + 1 + PM_KT + 1) * sizeof(double);
         if (lds > 64 * 1024) {
-            HS_HIP(hipFuncSetAttribute((const void*)potrf_block_mfma_kernel,
+            HS_HIP(hipFuncSetAttribute((const void*)potrf_block_mfma_kernel<double>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
-        potrf_block_mfma_kernel<<<1, 256, lds, h->stream>>>(A, N, b, blockID, blockRow, h->info_dev);
+        potrf_block_mfma_kernel<double><<<1, 256, lds, h->stream>>>(A, N, b, blockID, blockRow, h->info_dev);
         HS_LAUNCH_CHECK();
         return HS_OK;
     }
@@ -287,8 +295,8 @@ __global__ __launch_bounds__(1024) void trsm_panel_kernel(double* __restrict__ A
 // prefetched, so the per-step L access is a register read instead of a
 // strided global load; the solved column of the TR rows is broadcast through
 // LDS with 16-byte reads.
-template <int TR>
-__global__ __launch_bounds__(256) void trsm_panel_rowreg_kernel(double* __restrict__ A, int b, i64 blockID,
+template <int TR, typename T>
+__global__ __launch_bounds__(256) void trsm_panel_rowreg_kernel(T* __restrict__ A, int b, i64 blockID,
                                                                  int blockRow, int blockStart, int strips) {
     __shared__ double2 xs[2][TR / 2];
     const int g = blockIdx.x / strips;
@@ -296,16 +304,16 @@ __global__ __launch_bounds__(256) void trsm_panel_rowreg_kernel(double* __restri
     const int r0 = s * TR;
     const int t = threadIdx.x;
     const bool active = t < b;
-    const double* L = A + blockID * (i64)b * b;
-    const double* Lrow = L + (i64)(active ? t : 0) * b;
-    double* Bm = A + (blockID + (blockStart - blockRow) + g) * (i64)b * b;
+    const T* L = A + blockID * (i64)b * b;
+    const T* Lrow = L + (i64)(active ? t : 0) * b;
+    T* Bm = A + (blockID + (blockStart - blockRow) + g) * (i64)b * b;
     double x[TR];
 #pragma unroll
     for (int i = 0; i < TR; ++i) x[i] = active ? Bm[(i64)(r0 + i) * b + t] : 0.0;
     double lcur[16], lnext[16];
 #pragma unroll
     for (int j = 0; j < 16; j += 2) {
-        const double2 v = *reinterpret_cast<const double2*>(Lrow + j);
+        const double2 v = load2(Lrow + j);
         lcur[j] = v.x;
         lcur[j + 1] = v.y;
     }
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(256) void trsm_panel_rowreg_kernel(double* __restri
         if (c0 + 16 < b) {
 #pragma unroll
             for (int j = 0; j < 16; j += 2) {
-                const double2 v = *reinterpret_cast<const double2*>(Lrow + c0 + 16 + j);
+                const double2 v = load2(Lrow + c0 + 16 + j);
                 lnext[j] = v.x;
                 lnext[j + 1] = v.y;
             }
@@ -358,7 +366,7 @@ int hs_launch_trsm(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, i
         constexpr int TR = 16;
         const int strips = b / TR;
         const int threads = ((b + 63) / 64) * 64;
-        trsm_panel_rowreg_kernel<TR><<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(
+        trsm_panel_rowreg_kernel<TR, double><<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(
             A, b, blockID, blockRow, blockStart, strips);
         HS_LAUNCH_CHECK();
         return HS_OK;
@@ -753,4 +761,34 @@ extern "C" int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b
     int rc = hs_ensure_work(h, sizeof(double) * maxG);
     if (rc) return rc;
     return hs_launch_dot(h, x, y, (i64)blockStart * b, (i64)blockCount * b, (double*)h->work, maxG, result_dev);
+}
+
+// FP32-storage variants for the mixed-precision factor (arithmetic in FP64).
+int hs_launch_potrf_f32(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow) {
+    if (!(b % 32 == 0 && b <= 256)) {
+        hs_set_error("mixed-precision POTRF needs b % 32 == 0 and b <= 256");
+        return HS_ERR_UNSUPPORTED;
+    }
+    const size_t lds = (size_t)b * (PM_PW + 1 + PM_KT + 1) * sizeof(double);
+    if (lds > 64 * 1024) {
+        HS_HIP(hipFuncSetAttribute((const void*)potrf_block_mfma_kernel<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    potrf_block_mfma_kernel<float><<<1, 256, lds, h->stream>>>(A, N, b, blockID, blockRow, h->info_dev);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+int hs_launch_trsm_f32(hs_handle_t h, float* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount) {
+    if (blockCount <= 0) return HS_OK;
+    if (!(b % 16 == 0 && b <= 256)) {
+        hs_set_error("mixed-precision TRSM needs b % 16 == 0 and b <= 256");
+        return HS_ERR_UNSUPPORTED;
+    }
+    const int strips = b / 16;
+    const int threads = ((b + 63) / 64) * 64;
+    trsm_panel_rowreg_kernel<16, float><<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(
+        A, b, blockID, blockRow, blockStart, strips);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
 }

@@ -1,0 +1,271 @@
+// FP64 iterative refinement with a low-precision (FP32-stored) Cholesky
+// factor, and CG preconditioned by that factor when refinement stalls. No
+// counterpart in the reference.
+//
+// Every residual is r = y - K x on the FP64 K (read-once packed SYMV); the
+// corrections come from the blocked triangular solves on the FP32 factor in
+// FP64 arithmetic. Convergence is judged by the normwise backward error
+//   omega = ||r||_inf / (||K||_inf ||x||_inf + ||y||_inf),
+// with ||K||_inf from one pass of |K| times a vector of ones.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "hs_internal.h"
+
+int hs_symv_into(hs_handle_t h, const double* A, const double* x, double* y, const double* rhs, i64 N, int b,
+                 double* rowpart, double* colpart, double* part, int mode, bool abs_val);
+int hs_launch_dot(hs_handle_t h, const double* x, const double* y, i64 begin, i64 len, double* part, int maxG,
+                  double* out);
+
+namespace {
+
+constexpr int VT = 256;
+
+__global__ void fill_ones_kernel(double* __restrict__ v, i64 rows, i64 N) {
+    const i64 i = (i64)blockIdx.x * VT + threadIdx.x;
+    if (i < rows) v[i] = i < N ? 1.0 : 0.0;
+}
+
+__global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ c, i64 rows) {
+    const i64 i = (i64)blockIdx.x * VT + threadIdx.x;
+    if (i < rows) c[i] = a[i];
+}
+
+// y += alpha * x
+__global__ void axpy_kernel(double* __restrict__ y, const double* __restrict__ x, double alpha, i64 rows) {
+    const i64 i = (i64)blockIdx.x * VT + threadIdx.x;
+    if (i < rows) y[i] = y[i] + alpha * x[i];
+}
+
+// p = d + beta * p
+__global__ void xpby_kernel(const double* __restrict__ d, double* __restrict__ p, double beta, i64 rows) {
+    const i64 i = (i64)blockIdx.x * VT + threadIdx.x;
+    if (i < rows) p[i] = d[i] + beta * p[i];
+}
+
+// per-work-group max |u| and max |v| (order-independent, so deterministic)
+__global__ __launch_bounds__(VT) void maxabs2_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                                      i64 rows, double* __restrict__ part) {
+    __shared__ double ru[VT], rv[VT];
+    double mu = 0.0, mv = 0.0;
+    for (i64 i = (i64)blockIdx.x * VT + threadIdx.x; i < rows; i += (i64)gridDim.x * VT) {
+        mu = fmax(mu, fabs(u[i]));
+        mv = fmax(mv, fabs(v[i]));
+    }
+    ru[threadIdx.x] = mu;
+    rv[threadIdx.x] = mv;
+    __syncthreads();
+    for (int s = VT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ru[threadIdx.x] = fmax(ru[threadIdx.x], ru[threadIdx.x + s]);
+            rv[threadIdx.x] = fmax(rv[threadIdx.x], rv[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ru[0];
+        part[2 * blockIdx.x + 1] = rv[0];
+    }
+}
+
+__global__ __launch_bounds__(VT) void maxabs2_final_kernel(const double* __restrict__ part, int G,
+                                                            double* __restrict__ out) {
+    __shared__ double ru[VT], rv[VT];
+    double mu = 0.0, mv = 0.0;
+    for (int g = threadIdx.x; g < G; g += VT) {
+        mu = fmax(mu, part[2 * g]);
+        mv = fmax(mv, part[2 * g + 1]);
+    }
+    ru[threadIdx.x] = mu;
+    rv[threadIdx.x] = mv;
+    __syncthreads();
+    for (int s = VT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ru[threadIdx.x] = fmax(ru[threadIdx.x], ru[threadIdx.x + s]);
+            rv[threadIdx.x] = fmax(rv[threadIdx.x], rv[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = ru[0];
+        out[1] = rv[0];
+    }
+}
+
+struct Ctx {
+    hs_handle_t h;
+    const double* K;
+    const float* L;
+    i64 N;
+    int b, n;
+    i64 rows;
+    double *Dinv, *rowpart, *colpart, *r, *d, *z, *p, *q, *part, *scal;
+    double* pinned;
+    dim3 vg, vb;
+
+    int solve(double* v) { return hs_tri_solve_prepared<float>(h, L, Dinv, v, z, N, b); }
+    int residual(const double* y, const double* x) {  // r = y - K x
+        return hs_symv_into(h, K, x, r, y, N, b, rowpart, colpart, part, 2, false);
+    }
+    int maxabs2(const double* u, const double* v, double* mu, double* mv) {
+        const int G = (int)std::min<i64>(1024, (rows + VT - 1) / VT);
+        maxabs2_kernel<<<dim3((unsigned)G), vb, 0, h->stream>>>(u, v, rows, part);
+        HS_LAUNCH_CHECK();
+        maxabs2_final_kernel<<<1, vb, 0, h->stream>>>(part, G, scal);
+        HS_LAUNCH_CHECK();
+        HS_HIP(hipMemcpyAsync(pinned, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HS_HIP(hipStreamSynchronize(h->stream));
+        *mu = pinned[0];
+        *mv = pinned[1];
+        return HS_OK;
+    }
+    int dot(const double* u, const double* v, double* out) {
+        int rc = hs_launch_dot(h, u, v, 0, rows, part, 1024, scal + 2);
+        if (rc) return rc;
+        HS_HIP(hipMemcpyAsync(pinned, scal + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HS_HIP(hipStreamSynchronize(h->stream));
+        *out = pinned[0];
+        return HS_OK;
+    }
+    int copy(const double* a, double* c) {
+        copy_kernel<<<vg, vb, 0, h->stream>>>(a, c, rows);
+        HS_LAUNCH_CHECK();
+        return HS_OK;
+    }
+    int axpy(double* y, const double* x, double a) {
+        axpy_kernel<<<vg, vb, 0, h->stream>>>(y, x, a, rows);
+        HS_LAUNCH_CHECK();
+        return HS_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x,
+                               int64_t N, int b, double tol, int max_ir, int max_pcg, hs_refine_report* rep) {
+    if (!h) return -1;
+    if (!K) return -2;
+    if (!L32) return -3;
+    if (!y) return -4;
+    if (!x) return -5;
+    if (N <= 0) return -6;
+    if (!hs_trsv_fast_ok(b)) {
+        hs_set_error("hs_refine_solve: matrix_bsz must be 64, 128 or 256");
+        return -7;
+    }
+    if (!(tol > 0.0)) return -8;
+    if (max_ir < 0) return -9;
+    if (max_pcg < 0) return -10;
+    if (!rep) return -11;
+    Ctx c;
+    c.h = h;
+    c.K = K;
+    c.L = L32;
+    c.N = N;
+    c.b = b;
+    c.n = (int)((N + b - 1) / b);
+    c.rows = (i64)c.n * b;
+    const i64 nb = (i64)c.n * (c.n + 1) / 2;
+    const i64 G = c.rows / 64 + 2048 + 64;
+    const size_t doubles = (size_t)(2 * c.n * (i64)b * b + 2 * nb * b + 5 * c.rows + G + 16);
+    int rc = hs_ensure_work(h, doubles * sizeof(double));
+    if (rc) return rc;
+    double* w = (double*)h->work;
+    c.Dinv = w;
+    w += 2 * c.n * (i64)b * b;
+    c.rowpart = w;
+    w += nb * b;
+    c.colpart = w;
+    w += nb * b;
+    c.r = w;
+    w += c.rows;
+    c.d = w;
+    w += c.rows;
+    c.z = w;
+    w += c.rows;
+    c.p = w;
+    w += c.rows;
+    c.q = w;
+    w += c.rows;
+    c.part = w;
+    w += G;
+    c.scal = w;
+    c.pinned = (double*)h->pinned;
+    c.vg = dim3((unsigned)((c.rows + VT - 1) / VT));
+    c.vb = dim3(VT);
+    *rep = hs_refine_report{0, 0, 0, 0, 0.0, 0.0, 0.0};
+
+    if ((rc = hs_diag_inverse<float>(h, L32, N, b, c.Dinv))) return rc;
+    // ||K||_inf = max row sum of |K|, ||y||_inf
+    fill_ones_kernel<<<c.vg, c.vb, 0, h->stream>>>(c.d, c.rows, N);
+    HS_LAUNCH_CHECK();
+    if ((rc = hs_symv_into(h, K, c.d, c.q, nullptr, N, b, c.rowpart, c.colpart, c.part, 0, true))) return rc;
+    double knorm = 0.0, ynorm = 0.0;
+    if ((rc = c.maxabs2(c.q, y, &knorm, &ynorm))) return rc;
+    rep->knorm_inf = knorm;
+
+    // x0 = (L L^T)^{-1} y
+    if ((rc = c.copy(y, x)) || (rc = c.solve(x))) return rc;
+    double omega = 0.0, omegaPrev = 0.0, rmax, xmax;
+    bool converged = false, stalled = false;
+    for (int it = 0;; ++it) {
+        if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
+        omega = rmax / (knorm * xmax + ynorm);
+        if (omega <= tol) {
+            converged = true;
+            break;
+        }
+        if (it >= 1 && omega > 0.5 * omegaPrev) {
+            stalled = true;
+            break;
+        }
+        if (it >= max_ir) break;
+        omegaPrev = omega;
+        if ((rc = c.copy(c.r, c.d)) || (rc = c.solve(c.d)) || (rc = c.axpy(x, c.d, 1.0))) return rc;
+        rep->ir_iterations = it + 1;
+    }
+    if (!converged && max_pcg > 0) {
+        // r holds y - K x for the current x
+        rep->used_pcg = stalled || rep->ir_iterations >= max_ir ? 1 : 0;
+        double rho, pq;
+        if ((rc = c.copy(c.r, c.d)) || (rc = c.solve(c.d)) || (rc = c.copy(c.d, c.p)) || (rc = c.dot(c.r, c.d, &rho)))
+            return rc;
+        for (int k = 1; k <= max_pcg; ++k) {
+            if ((rc = hs_symv_into(h, K, c.p, c.q, nullptr, N, b, c.rowpart, c.colpart, c.part, 0, false))) return rc;
+            if ((rc = c.dot(c.p, c.q, &pq))) return rc;
+            const double alpha = rho / pq;
+            if ((rc = c.axpy(x, c.p, alpha)) || (rc = c.axpy(c.r, c.q, -alpha))) return rc;
+            rep->pcg_iterations = k;
+            if ((rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
+            if (rmax / (knorm * xmax + ynorm) <= tol) {
+                // confirm on the true residual before stopping
+                if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
+                omega = rmax / (knorm * xmax + ynorm);
+                if (omega <= tol) {
+                    converged = true;
+                    break;
+                }
+            }
+            double rhoNew;
+            if ((rc = c.copy(c.r, c.d)) || (rc = c.solve(c.d)) || (rc = c.dot(c.r, c.d, &rhoNew))) return rc;
+            xpby_kernel<<<c.vg, c.vb, 0, h->stream>>>(c.d, c.p, rhoNew / rho, c.rows);
+            HS_LAUNCH_CHECK();
+            rho = rhoNew;
+        }
+        if (!converged) {
+            if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
+            omega = rmax / (knorm * xmax + ynorm);
+        }
+    }
+    double rr = 0.0, yy = 0.0;
+    if ((rc = c.dot(c.r, c.r, &rr)) || (rc = c.dot(y, y, &yy))) return rc;
+    rep->backward_error = omega;
+    rep->relative_residual = std::sqrt(rr / yy);
+    rep->converged = omega <= tol ? 1 : 0;
+    if (!rep->converged) {
+        hs_set_error("hs_refine_solve: backward error target not reached");
+        return HS_ERR_NOT_CONVERGED;
+    }
+    return HS_OK;
+}

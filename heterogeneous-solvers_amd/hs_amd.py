@@ -64,7 +64,24 @@ _SIGS = {
     "hs_cg_f64": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],
     "hs_rbf_assemble_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P],
     "hs_rbf_window_f64": [_P, _P, _L, _I, _P],
+    "hs_to_f32_packed": [_P, _P, _P, _L, _I, _D],
+    "hs_cholesky_mixed": [_P, _P, _P, _L, _I, _I, _D],
+    "hs_cholesky_mixed_timed": [_P, _P, _P, _L, _I, _I, _D, C.POINTER(_D), C.POINTER(_L)],
+    "hs_refine_solve": [_P, _P, _P, _P, _P, _L, _I, _D, _I, _I, _P],
 }
+
+HS_MIXED_FP32 = 0
+HS_MIXED_BF16X3 = 1
+
+
+class RefineReport(C.Structure):
+    """hs_refine_report of include/hs_abi.h"""
+    _fields_ = [("ir_iterations", C.c_int64), ("pcg_iterations", C.c_int64), ("used_pcg", C.c_int32),
+                ("converged", C.c_int32), ("backward_error", C.c_double), ("relative_residual", C.c_double),
+                ("knorm_inf", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 _lib = None
 

@@ -174,6 +174,42 @@ int hs_rbf_assemble_f64(hs_handle_t h, const double* X, int64_t N, int d, int b,
  * sf2 = 1, ell = 1, sn2 = 0.01, as MatrixGenerator.cpp:77-99, 125-153. */
 int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* A);
 
+/* ---- mixed precision (no reference counterpart; the reference's nearest
+ * relative is its whole-program FP32 build, CMakeLists.txt:7, 104-106) ---- */
+
+#define HS_MIXED_FP32 0   /* trailing updates on FP32 MFMA */
+#define HS_MIXED_BF16X3 1 /* trailing updates on BF16 MFMA, hi+lo split, 3 products */
+
+/* Low-precision factor of the FP64 packed K: L32 (FP32 packed, same layout)
+ * <- chol(K + shift I). Diagonal POTRF and panel TRSM in FP64 arithmetic on
+ * FP32 storage, trailing SYRK/GEMM per `variant`, FP32 accumulation.
+ * matrix_bsz must be 128 or 256. Returns 0 or the first bad pivot (>0). */
+int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift);
+/* Same with per-phase kernel times (as hs_cholesky_f64_timed). */
+int hs_cholesky_mixed_timed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift,
+                            double* phase_ms, int64_t* launches);
+/* FP64 packed -> FP32 packed copy, shift added to the diagonal. */
+int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, double shift);
+
+typedef struct hs_refine_report_s {
+    int64_t ir_iterations;   /* refinement steps taken */
+    int64_t pcg_iterations;  /* PCG iterations after a refinement stall (0 if none) */
+    int32_t used_pcg;        /* 1 if refinement stalled and PCG finished the solve */
+    int32_t converged;       /* 1 if backward_error <= tol */
+    double backward_error;   /* ||y - Kx||_inf / (||K||_inf ||x||_inf + ||y||_inf) */
+    double relative_residual;/* ||y - Kx||_2 / ||y||_2 */
+    double knorm_inf;        /* ||K||_inf */
+} hs_refine_report;
+
+/* FP64 iterative refinement of K x = y with the low-precision factor L32:
+ * x0 = (L L^T)^{-1} y, then r = y - K x on the FP64 K, x += (L L^T)^{-1} r,
+ * until the normwise backward error is <= tol. When a step contracts the
+ * backward error by less than 2x, CG preconditioned by L L^T takes over from
+ * the current x (at most max_pcg iterations). Returns 0, or
+ * HS_ERR_NOT_CONVERGED with the report filled in. */
+int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N, int b,
+                    double tol, int max_ir, int max_pcg, hs_refine_report* report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+"""Mixed-precision factor (FP32 storage; FP32-MFMA or BF16 hi+lo trailing
+updates) and FP64 iterative refinement with the PCG fallback, against the
+FP64 oracle: the refined solution must reach the FP64 backward error target
+(1e-12) and agree with the oracle's FP64 solution to within what the
+conditioning allows."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hs_testlib import dense_to_packed, packed_len, packed_to_dense, synth_gp_numpy
+
+pytestmark = pytest.mark.gpu
+
+
+def system(oracle, N, d, b, ell, sn2, seed):
+    X, y = synth_gp_numpy(seed, N, d)
+    A = oracle.rbf(X, N, d, b, ell, 1.0, sn2)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    return A, rhs
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N,b", [(1024, 128), (1536, 256)])
+def test_mixed_factor_reconstructs(hs, dev, oracle, variant, N, b):
+    print("mixed factor variant=%d N=%d b=%d" % (variant, N, b), flush=True)
+    A, _ = system(oracle, N, 8, b, 1.0, 1e-1, 3)
+    dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
+    info = hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, variant, 0.0)
+    assert info == 0
+    L = dev.download(dL, packed_len(N, b), dtype=np.float32).astype(np.float64)
+    Ld = packed_to_dense(L, N, b, lower_only=True)
+    K = packed_to_dense(A, N, b)
+    err = np.linalg.norm(Ld @ Ld.T - K) / np.linalg.norm(K)
+    print("  ||LL^T - K|| / ||K|| = %.2e" % err)
+    # FP32 storage alone gives ~6e-8; the bf16 hi+lo products add ~2^-17
+    assert err < (2e-6 if variant == 0 else 2e-5)
+    # the factor agrees with the FP64 oracle's to low-precision accuracy
+    Lref = packed_to_dense(oracle.cholesky(A, N, b), N, b, lower_only=True)
+    assert np.max(np.abs(Ld - Lref)) < (1e-4 if variant == 0 else 1e-3) * np.max(np.abs(Lref))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N,b,ell,sn2", [(2048, 128, 1.0, 1e-2), (2048, 256, 2.0, 1e-3)])
+def test_refine_reaches_fp64_backward_error(hs, dev, oracle, variant, N, b, ell, sn2):
+    print("refine variant=%d N=%d b=%d" % (variant, N, b), flush=True)
+    A, rhs = system(oracle, N, 16, b, ell, sn2, 7)
+    dA, dL, dy, dx = dev.upload(A), dev.alloc(packed_len(N, b) * 4), dev.upload(rhs), dev.zeros(len(rhs))
+    assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, variant, 0.0) == 0
+    rep = hs.RefineReport()
+    hs.call("hs_refine_solve", dev.h, dA.p, dL.p, dy.p, dx.p, N, b, 1e-12, 30, 200, C.byref(rep))
+    x = dev.download(dx, len(rhs))
+    xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
+    K = packed_to_dense(A, N, b)
+    r = rhs[:N] - K @ x[:N]
+    omega = np.abs(r).max() / (np.abs(K).sum(1).max() * np.abs(x).max() + np.abs(rhs).max())
+    fwd = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    print("  report", rep.as_dict(), "host omega %.2e forward diff %.2e" % (omega, fwd))
+    assert rep.converged == 1 and rep.backward_error <= 1e-12
+    assert omega <= 2e-12
+    assert fwd <= 1e-8
+    assert np.all(x[N:] == 0.0)
+
+
+def test_pcg_fallback_and_shift(hs, dev, oracle):
+    """A shifted factor makes refinement contract slowly; the solve must switch
+    to PCG and still reach the target. max_ir = 0 goes to PCG at once."""
+    N, b = 2048, 128
+    A, rhs = system(oracle, N, 16, b, 2.0, 1e-3, 11)
+    K = packed_to_dense(A, N, b)
+    knorm = np.abs(K).sum(1).max()
+    dA, dL, dy, dx = dev.upload(A), dev.alloc(packed_len(N, b) * 4), dev.upload(rhs), dev.zeros(len(rhs))
+    for shift, max_ir in ((1e-3 * knorm, 30), (0.0, 0)):
+        assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, 0, shift) == 0
+        rep = hs.RefineReport()
+        hs.call("hs_refine_solve", dev.h, dA.p, dL.p, dy.p, dx.p, N, b, 1e-12, max_ir, 300, C.byref(rep))
+        print("  shift %.2e max_ir %d:" % (shift, max_ir), rep.as_dict())
+        assert rep.used_pcg == 1 and rep.pcg_iterations > 0 and rep.converged == 1
+        x = dev.download(dx, len(rhs))
+        xref = np.linalg.solve(K, rhs[:N])
+        assert np.linalg.norm(x[:N] - xref) <= 1e-8 * np.linalg.norm(xref)
+
+
+def test_mixed_reports_bad_pivot(hs, dev):
+    N, b = 512, 128
+    K = np.eye(N)
+    K[300, 300] = -2.0
+    A = dense_to_packed(K, b)
+    dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
+    assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, 1, 0.0) == 301
