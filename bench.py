@@ -129,6 +129,57 @@ def cholesky_workload(hs, dev, syn, args):
     }
 
 
+def mixed_workload(hs, dev, syn, args):
+    """Mixed precision on the same N = 65536 system: time to solution of
+    (FP32-stored factor + FP64 refinement to backward error 1e-12), for the
+    FP32-MFMA and the BF16 hi+lo trailing updates, K assembled beforehand."""
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
+    if args.N:
+        N = args.N
+    n = -(-N // b)
+    X, y = syn.gp_inputs(seed, N, d)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    plen = packed_len(N, b)
+    dX, dy, dx = dev.upload(X.reshape(-1)), dev.upload(rhs), dev.zeros(n * b)
+    dK = dev.alloc(plen * 8)
+    dL = dev.alloc(plen * 4)
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dK.p)
+    t_fac, t_ref = hs.Timer(dev), hs.Timer(dev)
+    phase = (C.c_double * 3)()
+    launches = (C.c_int64 * 3)()
+    out = {}
+    for name, variant in (("fp32", hs.HS_MIXED_FP32), ("bf16x3", hs.HS_MIXED_BF16X3)):
+        runs = []
+        for s in range(args.warmup + args.steps):
+            t_fac.start()
+            info = hs.call("hs_cholesky_mixed_timed", dev.h, dK.p, dL.p, N, b, variant, 0.0, phase, launches)
+            t_fac.stop()
+            if info != 0:
+                raise RuntimeError("mixed factorisation reported non-positive pivot %d" % info)
+            rep = hs.RefineReport()
+            t_ref.start()
+            rc = hs.load().hs_refine_solve(dev.h, dK.p, dL.p, dy.p, dx.p, N, b, 1e-12, 30, 300, C.byref(rep))
+            t_ref.stop()
+            if rc not in (0, hs.HS_ERR_NOT_CONVERGED):
+                raise hs.HsError(rc, "hs_refine_solve")
+            fac, ref = t_fac.ms(), t_ref.ms()
+            log("  mixed %s step %d: factor %.1f ms refine %.1f ms (%s)" % (name, s, fac, ref, rep.as_dict()))
+            if s >= args.warmup:
+                runs.append((fac, ref, list(phase), rep.as_dict()))
+        fac = float(np.median([r[0] for r in runs]))
+        ref = float(np.median([r[1] for r in runs]))
+        ph = np.mean([r[2] for r in runs], axis=0)
+        top = n - 2 if int(launches[2]) == n - 2 else n - 1
+        upd_flops = sum(j * j for j in range(1, top + 1)) * float(b) ** 3
+        out[name] = {"factor_ms": fac, "refine_ms": ref, "solve_ms": fac + ref, "report": runs[-1][3],
+                     "update_kernel_ms": float(ph[2]), "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
+                     "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}
+    dK.free()
+    dL.free()
+    return out
+
+
 def cg_workload(hs, dev, syn, args):
     seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cg"]
     n = -(-N // b)
@@ -203,6 +254,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=0, help="override N of the Cholesky workload (testing only)")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
     args = ap.parse_args()
@@ -215,7 +267,12 @@ def main():
     syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
     with hs.Device(0) as dev:
         ch = cholesky_workload(hs, dev, syn, args)
+        mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args)
         cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
+    if mixed:
+        fp64_solve = ch["factor_ms"] + ch["solve_ms"]
+        for v in mixed.values():
+            v["speedup_vs_fp64_factor_plus_solve"] = fp64_solve / v["solve_ms"]
     cpu = None if args.no_cpu else cpu_baseline(args)
     up = ch["update"]
     out = {
@@ -241,6 +298,7 @@ def main():
                      "launches": up["launches"]},
         "cpu_baseline": cpu,
         "cholesky": ch,
+        "mixed": mixed,
         "cg": cg,
     }
     print(json.dumps(out))

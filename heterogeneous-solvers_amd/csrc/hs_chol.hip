@@ -242,6 +242,7 @@ static int cholesky_run(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) 
     int rc = (b % 64 == 0) ? cholesky_lookahead(h, A, N, b, pt) : cholesky_sequential(h, A, N, b, pt);
     if (rc) {
         hipStreamSynchronize(h->stream2);
+        hipStreamSynchronize(h->stream_upd);
         return rc;
     }
     int64_t info = 0;

@@ -131,7 +131,8 @@ struct PhaseTimer {
 };
 
 // Right-looking factorisation with a look-ahead of one panel on two streams
-// of the same priority. Stream S2 factors panels: it applies step k's update
+// of the same priority (S1 = the handle's update stream, S2 = its panel
+// stream). Stream S2 factors panels: it applies step k's update
 // to block column k+1 only (trail mode 5), then POTRF and TRSM of panel k+1.
 // Stream S1 applies step k's update to the rest of the trailing triangle
 // (mode 4, J >= k+2) as soon as panel k is final, so the panel work of k+1
@@ -146,9 +147,12 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
     if (rc) return rc;
     hipEvent_t* evP = h->events.data();          // panel k final (S2)
     hipEvent_t* evU = h->events.data() + n + 1;  // S1 update k done
-    hipStream_t s1 = h->stream, s2 = h->stream2;
-    // S2 must see everything already queued on S1 (e.g. the input assembly)
-    HS_HIP(hipEventRecord(evU[n], s1));
+    // S1 is the CU-masked update stream; both streams join the handle's
+    // ordering stream at entry (e.g. after the input assembly) and the
+    // ordering stream joins S1 at exit
+    hipStream_t s0 = h->stream, s1 = h->stream_upd, s2 = h->stream2;
+    HS_HIP(hipEventRecord(evU[n], s0));
+    HS_HIP(hipStreamWaitEvent(s1, evU[n], 0));
     HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
     {
         OnStream os(h, s2);
@@ -174,5 +178,7 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
         HS_HIP(hipEventRecord(evU[k], s1));
     }
     HS_HIP(hipStreamWaitEvent(s1, evP[n - 1], 0));
+    HS_HIP(hipEventRecord(evP[n], s1));
+    HS_HIP(hipStreamWaitEvent(s0, evP[n], 0));
     return HS_OK;
 }

@@ -16,6 +16,12 @@ struct hs_handle_s {
     int device = 0;
     hipStream_t stream = nullptr;   // the handle's ordering stream
     hipStream_t stream2 = nullptr;  // look-ahead panel stream (same priority)
+    // trailing-update stream of the look-ahead factorisations: all CUs but
+    // one per 32 (one per XCD on MI355X), so the single-work-group panel
+    // kernels on stream2 always find a free CU; a plain stream when the
+    // runtime refuses the CU mask or HS_NO_CU_MASK is set
+    hipStream_t stream_upd = nullptr;
+    bool upd_masked = false;
     std::vector<hipEvent_t> events; // reusable ordering events (no timing)
     int* info_dev = nullptr;   // sticky first bad pivot (1-based global row), 0 = none
     // grow-only device workspace for partial sums / scalars

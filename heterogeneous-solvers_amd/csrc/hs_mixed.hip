@@ -342,6 +342,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     };
     if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
         hipStreamSynchronize(h->stream2);
+        hipStreamSynchronize(h->stream_upd);
         return rc;
     }
     int64_t info = 0;
