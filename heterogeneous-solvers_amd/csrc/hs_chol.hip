@@ -38,11 +38,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 
     const int tilesPerEdge = b / TILE;
     const int tilesPerBlock = tilesPerEdge * tilesPerEdge;
-    const i64 q = blockIdx.x / tilesPerBlock;
-    const int tile = blockIdx.x % tilesPerBlock;
+    int I, J, tile;
+    if (!tile_of(pm, tilesPerBlock, &I, &J, &tile)) return;
     const int tm = tile / tilesPerEdge, tn = tile % tilesPerEdge;
-    int I, J;
-    pair_of(pm, q, &I, &J);
     const bool diag = (I == J);
     if (diag && tm < tn) return;  // strictly upper tile of a diagonal block
 
@@ -122,31 +120,40 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
     }
     // epilogue: C -= acc. For the f64 16x16x4 MFMA, register v of lane l holds
     // row l/16 + 4v, column l%16 of the 16x16 tile (one row group per register,
-    // unlike the f32 variant's four consecutive rows).
-    if (!(diag && tm == tn)) {
+    // unlike the f32 variant's four consecutive rows). The C tile is read in
+    // batches of one 16-row group (MT x 4 values per lane) with the next group's
+    // loads issued before the current group's stores, since the compiler
+    // cannot prove the stores do not alias later loads and would otherwise
+    // wait for every load on its own. On the diagonal tile of a diagonal block
+    // only the lower triangle is written (the reference's SYRK writes i >= j
+    // only, MatrixMatrixOperations.cpp:273-299).
+    const bool dtile = diag && tm == tn;
+    double* pc = C + (i64)(wm0 + lq) * b + wn0 + lr;
+    double cv[MT][4], cn[MT][4];
 #pragma unroll
-        for (int i = 0; i < MT; ++i) {
+    for (int j = 0; j < MT; ++j)
 #pragma unroll
-            for (int j = 0; j < MT; ++j) {
-                double* p = C + (i64)(wm0 + i * 16 + lq) * b + wn0 + j * 16 + lr;
+        for (int v = 0; v < 4; ++v) cv[j][v] = pc[(i64)(4 * v) * b + j * 16];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) p[(i64)4 * v * b] -= acc[i][j][v];
-            }
+    for (int i = 0; i < MT; ++i) {
+        if (i + 1 < MT) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cn[j][v] = pc[(i64)((i + 1) * 16 + 4 * v) * b + j * 16];
         }
-    } else {
-        // diagonal tile of a diagonal block: lower triangle only (the
-        // reference's SYRK writes i >= j only, MatrixMatrixOperations.cpp:273-299)
 #pragma unroll
-        for (int i = 0; i < MT; ++i) {
+        for (int j = 0; j < MT; ++j)
 #pragma unroll
-            for (int j = 0; j < MT; ++j) {
-                const int col = wn0 + j * 16 + lr;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = wm0 + i * 16 + lq + 4 * v;
-                    if (row >= col) C[(i64)row * b + col] -= acc[i][j][v];
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm0 + i * 16 + lq + 4 * v, col = wn0 + j * 16 + lr;
+                if (!dtile || row >= col) pc[(i64)(i * 16 + 4 * v) * b + j * 16] = cv[j][v] - acc[i][j][v];
             }
+        if (i + 1 < MT) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cv[j][v] = cn[j][v];
         }
     }
 }
@@ -168,12 +175,9 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
     }
     const int tile = (b % 128 == 0) ? 128 : 64;
     const i64 tilesPerBlock = (i64)(b / tile) * (b / tile);
-    i64 pairs, off;
-    int rs;
-    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
-    if (pairs <= 0) return HS_OK;
-    PairMap pm{k, rs, mode, off};
-    const i64 grid = pairs * tilesPerBlock;
+    PairMap pm;
+    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, tilesPerBlock, &pm);
+    if (grid <= 0) return HS_OK;
     if (grid > 0x7fffffffLL) {
         hs_set_error("trailing update grid too large");
         return HS_ERR_UNSUPPORTED;
@@ -199,8 +203,10 @@ static int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    auto potrf = [&](int k) { return hs_launch_potrf(h, A, N, b, hs_block_id(k, k, n), k); };
-    auto trsm = [&](int k, int start, int count) { return hs_launch_trsm(h, A, b, hs_block_id(k, k, n), k, start, count); };
+    auto potrf = [&](int k) { return hs_factor_potrf<double>(h, A, N, b, hs_block_id(k, k, n), k); };
+    auto trsm = [&](int k, int start, int count) {
+        return hs_factor_trsm<double>(h, A, b, hs_block_id(k, k, n), k, start, count);
+    };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
         return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode);
     };
@@ -223,12 +229,12 @@ static int cholesky_sequential(hs_handle_t h, double* A, i64 N, int b, PhaseTime
     for (int k = 0; k < n; ++k) {
         const i64 id = hs_block_id(k, k, n);
         if ((rc = pt.begin(h->stream, 0))) return rc;
-        if ((rc = hs_launch_potrf(h, A, N, b, id, k))) return rc;
+        if ((rc = hs_factor_potrf<double>(h, A, N, b, id, k))) return rc;
         if ((rc = pt.end(h->stream, 0))) return rc;
         const int cnt = n - k - 1;
         if (cnt > 0) {
             if ((rc = pt.begin(h->stream, 1))) return rc;
-            if ((rc = hs_launch_trsm(h, A, b, id, k, k + 1, cnt))) return rc;
+            if ((rc = hs_factor_trsm<double>(h, A, b, id, k, k + 1, cnt))) return rc;
             if ((rc = pt.end(h->stream, 1))) return rc;
             if ((rc = pt.begin(h->stream, 2))) return rc;
             if ((rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) return rc;
@@ -242,7 +248,6 @@ static int cholesky_run(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) 
     int rc = (b % 64 == 0) ? cholesky_lookahead(h, A, N, b, pt) : cholesky_sequential(h, A, N, b, pt);
     if (rc) {
         hipStreamSynchronize(h->stream2);
-        hipStreamSynchronize(h->stream_upd);
         return rc;
     }
     int64_t info = 0;

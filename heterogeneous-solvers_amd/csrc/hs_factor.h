@@ -3,6 +3,8 @@
 // per-phase event timer and the one-panel look-ahead schedule.
 #pragma once
 
+#include <algorithm>
+#include <cstdlib>
 #include <utility>
 #include <vector>
 
@@ -19,6 +21,13 @@
 struct PairMap {
     int k, rowStart, mode;
     i64 pairOffset;
+    // band order (modes 3 and 4 over a whole triangle): m rows, bands of
+    // `band` block rows walked column by column; m == 0 keeps pair_of's order
+    int m = 0, band = 4;
+    // XCD chunking: work-group g runs logical tile (g % 8) * per + g / 8, so
+    // each XCD (work-groups are dealt to the 8 XCDs round-robin) walks one
+    // contiguous stretch of the order; per == 0 keeps the launch order
+    i64 total = 0, per = 0;
 };
 
 __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
@@ -49,6 +58,59 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
     }
 }
 
+// Pair q of a triangle of m block rows (relative rows i >= columns j) in
+// band order: bands of H rows; inside a band, column j by column j, the
+// band's rows below the diagonal. Consecutive work-groups then share the
+// B operand L_Jk while the band's H A operands stay in the XCD's L2, so an
+// operand block is fetched once per band instead of once per row.
+__device__ inline void band_decode(i64 q, int m, int H, int* pi, int* pj) {
+    auto S = [&](i64 B) { return (i64)H * H * B * (B - 1) / 2 + B * (i64)H * (H + 1) / 2; };
+    int lo = 0, hi = (m + H - 1) / H - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S(mid) <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    i64 p = q - S(lo);
+    const int r0 = lo * H;
+    const int hp = min(H, m - r0);
+    const i64 rect = (i64)hp * r0;
+    if (p < rect) {
+        *pj = (int)(p / hp);
+        *pi = r0 + (int)(p % hp);
+        return;
+    }
+    p -= rect;
+    int t = 0;
+    while (p >= hp - t) {
+        p -= hp - t;
+        ++t;
+    }
+    *pj = r0 + t;
+    *pi = r0 + t + (int)p;
+}
+
+// Block pair (I, J) and tile index of this work-group; false for the idle
+// work-groups that pad an XCD-chunked launch to a multiple of 8.
+__device__ inline bool tile_of(const PairMap& pm, int tilesPerBlock, int* I, int* J, int* tile) {
+    i64 g = blockIdx.x;
+    if (pm.per > 0) {
+        g = (g % 8) * pm.per + g / 8;
+        if (g >= pm.total) return false;
+    }
+    const i64 q = g / tilesPerBlock;
+    *tile = (int)(g % tilesPerBlock);
+    if (pm.m > 0) {
+        int i, j;
+        band_decode(q, pm.m, pm.band, &i, &j);
+        *I = pm.rowStart + i;
+        *J = pm.rowStart + j;
+    } else {
+        pair_of(pm, q, I, J);
+    }
+    return true;
+}
+
 // Number of block pairs and the pair offset of a launch (see PairMap).
 inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, i64* pairs, i64* offset, int* rs) {
     *offset = 0;
@@ -77,6 +139,44 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, i6
         *pairs = r1 * (r1 + 1) / 2 - r0 * (r0 + 1) / 2;
         *offset = r0 * (r0 + 1) / 2;
     }
+}
+
+// Work-group order of a trailing-update launch; HS_TRAIL_ORDER (bit 0 band
+// order, bit 1 XCD chunking, default both) and HS_TRAIL_BAND (band height,
+// default 4) select it for experiments.
+inline void hs_trail_order(int* bits, int* band) {
+    static int b0 = -1, h0 = 4;
+    if (b0 < 0) {
+        const char* e = std::getenv("HS_TRAIL_ORDER");
+        b0 = e ? std::atoi(e) : 3;
+        const char* hb = std::getenv("HS_TRAIL_BAND");
+        h0 = hb ? std::max(1, std::atoi(hb)) : 4;
+    }
+    *bits = b0;
+    *band = h0;
+}
+
+// Fills the pair map of a trailing-update launch with tilesPerBlock work-
+// groups per block pair; returns the grid size (0: nothing to do).
+inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, i64 tilesPerBlock, PairMap* pm) {
+    i64 pairs, off;
+    int rs;
+    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
+    *pm = PairMap{k, rs, mode, off};
+    if (pairs <= 0) return 0;
+    int bits, band;
+    hs_trail_order(&bits, &band);
+    if ((bits & 1) && (mode == 4 || (mode == 3 && off == 0))) {
+        pm->m = (mode == 4) ? n - k - 2 : rowCount;
+        pm->band = band;
+    }
+    const i64 total = pairs * tilesPerBlock;
+    if ((bits & 2) && total >= 64) {
+        pm->total = total;
+        pm->per = (total + 7) / 8;
+        return 8 * pm->per;
+    }
+    return total;
 }
 
 // Runs launches that use h->stream on another stream of the same handle.
@@ -131,8 +231,7 @@ struct PhaseTimer {
 };
 
 // Right-looking factorisation with a look-ahead of one panel on two streams
-// of the same priority (S1 = the handle's update stream, S2 = its panel
-// stream). Stream S2 factors panels: it applies step k's update
+// of the same priority. Stream S2 factors panels: it applies step k's update
 // to block column k+1 only (trail mode 5), then POTRF and TRSM of panel k+1.
 // Stream S1 applies step k's update to the rest of the trailing triangle
 // (mode 4, J >= k+2) as soon as panel k is final, so the panel work of k+1
@@ -147,12 +246,9 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
     if (rc) return rc;
     hipEvent_t* evP = h->events.data();          // panel k final (S2)
     hipEvent_t* evU = h->events.data() + n + 1;  // S1 update k done
-    // S1 is the CU-masked update stream; both streams join the handle's
-    // ordering stream at entry (e.g. after the input assembly) and the
-    // ordering stream joins S1 at exit
-    hipStream_t s0 = h->stream, s1 = h->stream_upd, s2 = h->stream2;
-    HS_HIP(hipEventRecord(evU[n], s0));
-    HS_HIP(hipStreamWaitEvent(s1, evU[n], 0));
+    hipStream_t s1 = h->stream, s2 = h->stream2;
+    // S2 must see everything already queued on S1 (e.g. the input assembly)
+    HS_HIP(hipEventRecord(evU[n], s1));
     HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
     {
         OnStream os(h, s2);
@@ -178,7 +274,5 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
         HS_HIP(hipEventRecord(evU[k], s1));
     }
     HS_HIP(hipStreamWaitEvent(s1, evP[n - 1], 0));
-    HS_HIP(hipEventRecord(evP[n], s1));
-    HS_HIP(hipStreamWaitEvent(s0, evP[n], 0));
     return HS_OK;
 }

@@ -16,12 +16,6 @@ struct hs_handle_s {
     int device = 0;
     hipStream_t stream = nullptr;   // the handle's ordering stream
     hipStream_t stream2 = nullptr;  // look-ahead panel stream (same priority)
-    // trailing-update stream of the look-ahead factorisations: all CUs but
-    // one per 32 (one per XCD on MI355X), so the single-work-group panel
-    // kernels on stream2 always find a free CU; a plain stream when the
-    // runtime refuses the CU mask or HS_NO_CU_MASK is set
-    hipStream_t stream_upd = nullptr;
-    bool upd_masked = false;
     std::vector<hipEvent_t> events; // reusable ordering events (no timing)
     int* info_dev = nullptr;   // sticky first bad pivot (1-based global row), 0 = none
     // grow-only device workspace for partial sums / scalars
@@ -30,6 +24,10 @@ struct hs_handle_s {
     // pinned host word used to poll device-side loop state without a full copy
     void* pinned = nullptr;
     int cu_count = 256;
+    // inverses of the 16x16 diagonal sub-blocks of the last factored diagonal
+    // block (up to 16 of them, b <= 256), written by the panel POTRF, read by
+    // the panel TRSM that follows it on the same stream
+    double* trsm_w = nullptr;
 };
 
 struct hs_event_s {
@@ -75,6 +73,23 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
 // column k+1, rows k+1..n-1) split step k for the look-ahead driver
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
                           int mode);
+// panel kernels for b % 32 == 0 (POTRF) / b % 16 == 0 (TRSM), b <= 256;
+// W = the 16x16 diagonal sub-block inverses of the factor (POTRF writes them
+// when W is non-null; a TRSM given W == nullptr computes them into trsm_w)
+bool hs_potrf_fast_ok(int b);
+bool hs_trsm_fast_ok(int b);
+template <typename ST>
+int hs_launch_potrf_fast(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, double* W);
+template <typename ST>
+int hs_launch_trsm_fast(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount,
+                        const double* W);
+// factorisation steps: the panel kernels where b allows them (the POTRF then
+// leaves the sub-block inverses for the TRSM of the same column), else the
+// element-wise kernels (FP64 storage only)
+template <typename ST>
+int hs_factor_potrf(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow);
+template <typename ST>
+int hs_factor_trsm(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
 // fast blocked triangular solves (b in {64, 128, 256}); factor in FP64 or FP32
 bool hs_trsv_fast_ok(int b);
 template <typename T>

@@ -18,8 +18,6 @@
 #include "hs_factor.h"
 #include "hs_internal.h"
 
-int hs_launch_potrf_f32(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow);
-int hs_launch_trsm_f32(hs_handle_t h, float* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
 
 namespace {
 
@@ -44,6 +42,45 @@ __global__ void diag_shift_kernel(const double* __restrict__ K, float* __restric
 }
 
 // ---- FP32 MFMA trailing update -------------------------------------------
+// C -= acc for one work-group tile (f32 MFMA layout: register v of lane l is
+// row 4(l/16) + v, column l%16). The tile is read in batches of one 16-row
+// group with the next group's loads issued before the current group's
+// stores: the compiler cannot prove the stores do not alias later loads and
+// would otherwise wait for every load on its own. On the diagonal tile of a
+// diagonal block only the lower triangle is written.
+template <int MT>
+__device__ inline void epilogue_f32(float* C, int b, int wm0, int wn0, int lr, int lq, bool dtile,
+                                    const f4 (&acc)[MT][MT]) {
+    float* pc = C + (i64)(wm0 + 4 * lq) * b + wn0 + lr;
+    float cv[MT][4], cn[MT][4];
+#pragma unroll
+    for (int j = 0; j < MT; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cv[j][v] = pc[(i64)v * b + j * 16];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+        if (i + 1 < MT) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cn[j][v] = pc[(i64)((i + 1) * 16 + v) * b + j * 16];
+        }
+#pragma unroll
+        for (int j = 0; j < MT; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm0 + i * 16 + 4 * lq + v, col = wn0 + j * 16 + lr;
+                if (!dtile || row >= col) pc[(i64)(i * 16 + v) * b + j * 16] = cv[j][v] - acc[i][j][v];
+            }
+        if (i + 1 < MT) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cv[j][v] = cn[j][v];
+        }
+    }
+}
+
 // TILE x TILE output tile per work-group, 2x2 waves, 16x16x4 f32 MFMA tiles
 // (register v of lane l: row 4(l/16)+v, column l%16), KT = 16 deep LDS
 // stages, double buffered, operands stored k-major.
@@ -56,11 +93,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
     __shared__ float As[2 * STAGE];
     __shared__ float Bs[2 * STAGE];
     const int tpb = (b / TILE) * (b / TILE);
-    const i64 q = blockIdx.x / tpb;
-    const int tile = blockIdx.x % tpb;
+    int I, J, tile;
+    if (!tile_of(pm, tpb, &I, &J, &tile)) return;
     const int tm = tile / (b / TILE), tn = tile % (b / TILE);
-    int I, J;
-    pair_of(pm, q, &I, &J);
     const bool diag = (I == J);
     if (diag && tm < tn) return;
     const i64 bb = (i64)b * b;
@@ -130,18 +165,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
         }
         __syncthreads();
     }
-    const bool dtile = diag && tm == tn;
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int j = 0; j < MT; ++j) {
-            const int col = wn0 + j * 16 + lr;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = wm0 + i * 16 + 4 * lq + v;
-                if (!dtile || row >= col) C[(i64)row * b + col] -= acc[i][j][v];
-            }
-        }
+    epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
 // ---- BF16 hi+lo split trailing update -------------------------------------
@@ -165,11 +189,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     constexpr int STAGE = 4 * PLANE;      // A hi, A lo, B hi, B lo
     __shared__ unsigned short lds[2 * STAGE];
     const int tpb = (b / TILE) * (b / TILE);
-    const i64 q = blockIdx.x / tpb;
-    const int tile = blockIdx.x % tpb;
+    int I, J, tile;
+    if (!tile_of(pm, tpb, &I, &J, &tile)) return;
     const int tm = tile / (b / TILE), tn = tile % (b / TILE);
-    int I, J;
-    pair_of(pm, q, &I, &J);
     const bool diag = (I == J);
     if (diag && tm < tn) return;
     const i64 bb = (i64)b * b;
@@ -265,18 +287,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
         }
         __syncthreads();
     }
-    const bool dtile = diag && tm == tn;
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int j = 0; j < MT; ++j) {
-            const int col = wn0 + j * 16 + lr;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = wm0 + i * 16 + 4 * lq + v;
-                if (!dtile || row >= col) C[(i64)row * b + col] -= acc[i][j][v];
-            }
-        }
+    epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
 }  // namespace
@@ -290,12 +301,9 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     }
     constexpr int TILE = 128;
     const i64 tpb = (i64)(b / TILE) * (b / TILE);
-    i64 pairs, off;
-    int rs;
-    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
-    if (pairs <= 0) return HS_OK;
-    PairMap pm{k, rs, mode, off};
-    const i64 grid = pairs * tpb;
+    PairMap pm;
+    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, tpb, &pm);
+    if (grid <= 0) return HS_OK;
     if (grid > 0x7fffffffLL) {
         hs_set_error("trailing update grid too large");
         return HS_ERR_UNSUPPORTED;
@@ -333,16 +341,15 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    auto potrf = [&](int k) { return hs_launch_potrf_f32(h, L32, N, b, hs_block_id(k, k, n), k); };
+    auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
     auto trsm = [&](int k, int start, int count) {
-        return hs_launch_trsm_f32(h, L32, b, hs_block_id(k, k, n), k, start, count);
+        return hs_factor_trsm<float>(h, L32, b, hs_block_id(k, k, n), k, start, count);
     };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
         return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant);
     };
     if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
         hipStreamSynchronize(h->stream2);
-        hipStreamSynchronize(h->stream_upd);
         return rc;
     }
     int64_t info = 0;

@@ -92,145 +92,9 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_block_kernel(double* __re
     }
 }
 
-// Fast POTRF for b % 32 == 0, b <= 256: the same left-looking panel scheme
-// with 32-wide panels; the update of a panel by the finished columns,
-// P -= T T_c^T, runs on v_mfma_f64_16x16x4f64 (wave w owns row tiles
-// w, w+4, ... of the panel, both 16-column tiles), and the next 16-deep
-// chunk of finished columns is prefetched into registers while the current
-// chunk feeds the MFMAs.
-typedef double d4v __attribute__((ext_vector_type(4)));
-
-// two consecutive stored values as doubles (16-byte or 8-byte load)
-__device__ inline double2 load2(const double* p) { return *reinterpret_cast<const double2*>(p); }
-__device__ inline double2 load2(const float* p) {
-    const float2 f = *reinterpret_cast<const float2*>(p);
-    return make_double2((double)f.x, (double)f.y);
-}
-constexpr int PM_PW = 32;
-constexpr int PM_KT = 16;
-
-template <typename ST>
-__global__ __launch_bounds__(256) void potrf_block_mfma_kernel(ST* __restrict__ A, i64 N, int b, i64 blockID,
-                                                                int blockRow, int* __restrict__ info) {
-    extern __shared__ double smem[];
-    constexpr int PP = PM_PW + 1;  // panel pitch
-    constexpr int TP = PM_KT + 1;  // chunk pitch
-    double* P = smem;              // [b][PP]
-    double* T = smem + (size_t)b * PP;  // [b][TP]
-    ST* B = A + blockID * (i64)b * b;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int lr = lane & 15, lq = lane >> 4;
-    const i64 rowBase = (i64)blockRow * b;
-    const int nvalid = (int)std::min<i64>((i64)b, std::max<i64>((i64)0, N - rowBase));
-
-    for (int p = 0; p < b; p += PM_PW) {
-        const int rows = b - p;
-        for (int e = tid; e < rows * PM_PW; e += 256) {
-            const int r = e / PM_PW, c = e % PM_PW;
-            P[r * PP + c] = (r >= c) ? B[(i64)(p + r) * b + p + c] : 0.0;
-        }
-        if (p > 0) {
-            const int rtiles = rows / 16;
-            d4v acc[4][2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = d4v{0.0, 0.0, 0.0, 0.0};
-            // chunk staging: element e = tid + 256 i -> row e/8, column pair (e%8)*2
-            double2 pre[8];
-            auto gload = [&](int k0) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int e = tid + 256 * i;
-                    const int r = e >> 3, kk = (e & 7) * 2;
-                    if (r < rows) pre[i] = load2(B + (i64)(p + r) * b + k0 + kk);
-                }
-            };
-            gload(0);
-            for (int k0 = 0; k0 < p; k0 += PM_KT) {
-                __syncthreads();  // previous chunk consumed (and P loaded)
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int e = tid + 256 * i;
-                    const int r = e >> 3, kk = (e & 7) * 2;
-                    if (r < rows) {
-                        T[r * TP + kk] = pre[i].x;
-                        T[r * TP + kk + 1] = pre[i].y;
-                    }
-                }
-                __syncthreads();
-                if (k0 + PM_KT < p) gload(k0 + PM_KT);
-#pragma unroll
-                for (int s = 0; s < PM_KT; s += 4) {
-                    const double b0 = T[(lr)*TP + s + lq];
-                    const double b1 = T[(16 + lr) * TP + s + lq];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int rt = wave + 4 * i;
-                        if (rt < rtiles) {
-                            const double a = T[(rt * 16 + lr) * TP + s + lq];
-                            acc[i][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[i][0], 0, 0, 0);
-                            acc[i][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[i][1], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-            // subtract: register v of lane l holds row l/16 + 4v, column l%16
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int rt = wave + 4 * i;
-                if (rt < rtiles) {
-#pragma unroll
-                    for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const int r = rt * 16 + lq + 4 * v, c = ct * 16 + lr;
-                            if (r >= c && p + r < nvalid) P[r * PP + c] -= acc[i][ct][v];
-                        }
-                }
-            }
-        }
-        __syncthreads();
-        // unblocked right-looking factorisation of the tall panel
-        for (int c = 0; c < PM_PW; ++c) {
-            const double d = P[c * PP + c];
-            const double s = sqrt(d);
-            if (tid == 0 && (p + c) < nvalid && !(d > 0.0)) atomicCAS(info, 0, (int)(rowBase + p + c + 1));
-            for (int r = c + 1 + tid; r < rows; r += 256)
-                if (p + r < nvalid) P[r * PP + c] /= s;
-            __syncthreads();
-            if (tid == 0) P[c * PP + c] = s;
-            const int w = PM_PW - c - 1;
-            for (int e = tid; e < rows * w; e += 256) {
-                const int r = e / w, cc = c + 1 + e % w;
-                if (r >= cc && p + r < nvalid) P[r * PP + cc] -= P[r * PP + c] * P[cc * PP + c];
-            }
-            __syncthreads();
-        }
-        for (int e = tid; e < rows * PM_PW; e += 256) {
-            const int r = e / PM_PW, c = e % PM_PW;
-            const int gi = p + r;
-            if (gi < nvalid) B[(i64)gi * b + p + c] = (r >= c) ? P[r * PP + c] : 0.0;
-            else if (r == c) B[(i64)gi * b + p + c] = P[r * PP + c];
-        }
-        for (int e = tid; e < PM_PW * b; e += 256) {
-            const int r = e / b, j = e % b;
-            if (j >= p + PM_PW && p + r < nvalid) B[(i64)(p + r) * b + j] = 0.0;
-        }
-        __syncthreads();
-    }
-}
-
+// Panel kernel (hs_panel.hip) for the solver block sizes, else the generic one.
 int hs_launch_potrf(hs_handle_t h, double* A, i64 N, int b, i64 blockID, int blockRow) {
-    if (b % 32 == 0 && b <= 256) {
-        const size_t lds = (size_t)b * (PM_PW + 1 + PM_KT + 1) * sizeof(double);
-        if (lds > 64 * 1024) {
-            HS_HIP(hipFuncSetAttribute((const void*)potrf_block_mfma_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        }
-        potrf_block_mfma_kernel<double><<<1, 256, lds, h->stream>>>(A, N, b, blockID, blockRow, h->info_dev);
-        HS_LAUNCH_CHECK();
-        return HS_OK;
-    }
+    if (hs_potrf_fast_ok(b)) return hs_launch_potrf_fast<double>(h, A, N, b, blockID, blockRow, nullptr);
     const size_t lds = (size_t)b * (POTRF_PW + 1 + POTRF_KT + 1) * sizeof(double);
     if (lds > 160 * 1024) {
         hs_set_error("POTRF: block size too large for one work-group's LDS (b <= 512)");
@@ -290,87 +154,13 @@ __global__ __launch_bounds__(1024) void trsm_panel_kernel(double* __restrict__ A
     }
 }
 
-// Same algorithm for b a multiple of 16 (the solver block sizes): thread t
-// keeps its own row of L in registers, 16 columns at a time with the next 16
-// prefetched, so the per-step L access is a register read instead of a
-// strided global load; the solved column of the TR rows is broadcast through
-// LDS with 16-byte reads.
-template <int TR, typename T>
-__global__ __launch_bounds__(256) void trsm_panel_rowreg_kernel(T* __restrict__ A, int b, i64 blockID,
-                                                                 int blockRow, int blockStart, int strips) {
-    __shared__ double2 xs[2][TR / 2];
-    const int g = blockIdx.x / strips;
-    const int s = blockIdx.x % strips;
-    const int r0 = s * TR;
-    const int t = threadIdx.x;
-    const bool active = t < b;
-    const T* L = A + blockID * (i64)b * b;
-    const T* Lrow = L + (i64)(active ? t : 0) * b;
-    T* Bm = A + (blockID + (blockStart - blockRow) + g) * (i64)b * b;
-    double x[TR];
-#pragma unroll
-    for (int i = 0; i < TR; ++i) x[i] = active ? Bm[(i64)(r0 + i) * b + t] : 0.0;
-    double lcur[16], lnext[16];
-#pragma unroll
-    for (int j = 0; j < 16; j += 2) {
-        const double2 v = load2(Lrow + j);
-        lcur[j] = v.x;
-        lcur[j + 1] = v.y;
-    }
-    for (int c0 = 0; c0 < b; c0 += 16) {
-        if (c0 + 16 < b) {
-#pragma unroll
-            for (int j = 0; j < 16; j += 2) {
-                const double2 v = load2(Lrow + c0 + 16 + j);
-                lnext[j] = v.x;
-                lnext[j + 1] = v.y;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int c = c0 + j;
-            const int buf = c & 1;
-            if (t == c) {
-#pragma unroll
-                for (int i = 0; i < TR; ++i) x[i] = x[i] / lcur[j];
-#pragma unroll
-                for (int i = 0; i < TR / 2; ++i) xs[buf][i] = make_double2(x[2 * i], x[2 * i + 1]);
-            }
-            __syncthreads();
-            if (t > c && active) {
-                const double ltc = lcur[j];
-#pragma unroll
-                for (int i = 0; i < TR / 2; ++i) {
-                    const double2 v = xs[buf][i];
-                    x[2 * i] = x[2 * i] - ltc * v.x;
-                    x[2 * i + 1] = x[2 * i + 1] - ltc * v.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) lcur[j] = lnext[j];
-    }
-    if (active) {
-#pragma unroll
-        for (int i = 0; i < TR; ++i) Bm[(i64)(r0 + i) * b + t] = x[i];
-    }
-}
-
 int hs_launch_trsm(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount) {
     if (blockCount <= 0) return HS_OK;
     if (b > 1024) {
         hs_set_error("TRSM: block size above 1024 is not supported");
         return HS_ERR_UNSUPPORTED;
     }
-    if (b % 16 == 0 && b <= 256) {
-        constexpr int TR = 16;
-        const int strips = b / TR;
-        const int threads = ((b + 63) / 64) * 64;
-        trsm_panel_rowreg_kernel<TR, double><<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(
-            A, b, blockID, blockRow, blockStart, strips);
-        HS_LAUNCH_CHECK();
-        return HS_OK;
-    }
+    if (hs_trsm_fast_ok(b)) return hs_launch_trsm_fast<double>(h, A, b, blockID, blockRow, blockStart, blockCount, nullptr);
     const int strips = (b + TRSM_TR - 1) / TRSM_TR;
     const int threads = ((b + 63) / 64) * 64;
     trsm_panel_kernel<<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(A, b, blockID, blockRow,
@@ -763,32 +553,3 @@ extern "C" int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b
     return hs_launch_dot(h, x, y, (i64)blockStart * b, (i64)blockCount * b, (double*)h->work, maxG, result_dev);
 }
 
-// FP32-storage variants for the mixed-precision factor (arithmetic in FP64).
-int hs_launch_potrf_f32(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow) {
-    if (!(b % 32 == 0 && b <= 256)) {
-        hs_set_error("mixed-precision POTRF needs b % 32 == 0 and b <= 256");
-        return HS_ERR_UNSUPPORTED;
-    }
-    const size_t lds = (size_t)b * (PM_PW + 1 + PM_KT + 1) * sizeof(double);
-    if (lds > 64 * 1024) {
-        HS_HIP(hipFuncSetAttribute((const void*)potrf_block_mfma_kernel<float>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
-    potrf_block_mfma_kernel<float><<<1, 256, lds, h->stream>>>(A, N, b, blockID, blockRow, h->info_dev);
-    HS_LAUNCH_CHECK();
-    return HS_OK;
-}
-
-int hs_launch_trsm_f32(hs_handle_t h, float* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount) {
-    if (blockCount <= 0) return HS_OK;
-    if (!(b % 16 == 0 && b <= 256)) {
-        hs_set_error("mixed-precision TRSM needs b % 16 == 0 and b <= 256");
-        return HS_ERR_UNSUPPORTED;
-    }
-    const int strips = b / 16;
-    const int threads = ((b + 63) / 64) * 64;
-    trsm_panel_rowreg_kernel<16, float><<<dim3((unsigned)(blockCount * strips)), dim3(threads), 0, h->stream>>>(
-        A, b, blockID, blockRow, blockStart, strips);
-    HS_LAUNCH_CHECK();
-    return HS_OK;
-}

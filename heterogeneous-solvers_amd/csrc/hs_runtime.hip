@@ -2,7 +2,6 @@
 // This is the stand-in for the reference's sycl::queue + USM calls
 // (main.cpp:189-192, Cholesky.cpp:24-32, CG.cpp:165-234).
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -45,31 +44,11 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cu_count = prop.multiProcessorCount;
-    {
-        const char* off = std::getenv("HS_NO_CU_MASK");
-        if (!(off && off[0] == '1') && h->cu_count >= 64) {
-            std::vector<uint32_t> mask((h->cu_count + 31) / 32, 0xFFFFFFFFu);
-            for (auto& m : mask) m &= ~1u;  // leave CU 32*i to the panel stream
-            if (hipExtStreamCreateWithCUMask(&h->stream_upd, (uint32_t)mask.size(), mask.data()) == hipSuccess)
-                h->upd_masked = true;
-            else
-                h->stream_upd = nullptr;
-        }
-        if (!h->stream_upd) {
-            e = hipStreamCreateWithFlags(&h->stream_upd, hipStreamNonBlocking);
-            if (e != hipSuccess) {
-                hipStreamDestroy(h->stream2);
-                hipStreamDestroy(h->stream);
-                delete h;
-                return hs_hip_fail(e, "hipStreamCreateWithFlags");
-            }
-        }
-    }
     e = hipMalloc((void**)&h->info_dev, sizeof(int) * 2);
     if (e == hipSuccess) e = hipMemset(h->info_dev, 0, sizeof(int) * 2);
     if (e == hipSuccess) e = hipHostMalloc(&h->pinned, 4096, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->trsm_w, sizeof(double) * 16 * 256);
     if (e != hipSuccess) {
-        hipStreamDestroy(h->stream_upd);
         hipStreamDestroy(h->stream2);
         hipStreamDestroy(h->stream);
         delete h;
@@ -84,12 +63,11 @@ extern "C" int hs_handle_destroy(hs_handle_t h) {
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
     hipStreamSynchronize(h->stream2);
-    hipStreamSynchronize(h->stream_upd);
     for (auto e : h->events) hipEventDestroy(e);
-    hipStreamDestroy(h->stream_upd);
     hipStreamDestroy(h->stream2);
     if (h->work) hipFree(h->work);
     if (h->info_dev) hipFree(h->info_dev);
+    if (h->trsm_w) hipFree(h->trsm_w);
     if (h->pinned) hipHostFree(h->pinned);
     hipStreamDestroy(h->stream);
     delete h;
