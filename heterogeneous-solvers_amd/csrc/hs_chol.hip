@@ -312,3 +312,46 @@ extern "C" int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64
     }
     return HS_OK;
 }
+
+// One triangular solve in place on x: L x' = x (trans 0) or L^T x' = x
+// (trans 1); one half of TriangularSystemSolver::solve
+// (TriangularSystemSolver.cpp:31-56 forward, :58-82 backward).
+extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b, int trans) {
+    if (!h) return -1;
+    if (!L) return -2;
+    if (!x) return -3;
+    if (N <= 0) return -4;
+    if (b <= 0) return -5;
+    if (trans != 0 && trans != 1) return -6;
+    const int n = (int)((N + b - 1) / b);
+    int rc;
+    if (hs_trsv_fast_ok(b)) {
+        const size_t nb = (size_t)n * b;
+        if ((rc = hs_ensure_work(h, sizeof(double) * ((size_t)2 * n * b * b + nb)))) return rc;
+        double* Dinv = (double*)h->work;
+        double* z = Dinv + (size_t)2 * n * b * b;
+        if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
+        if (trans == 0) {
+            if ((rc = hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b, 1))) return rc;
+            HS_HIP(hipMemcpyAsync(x, z, sizeof(double) * nb, hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            HS_HIP(hipMemcpyAsync(z, x, sizeof(double) * nb, hipMemcpyDeviceToDevice, h->stream));
+            if ((rc = hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b, 2))) return rc;
+        }
+        return HS_OK;
+    }
+    if (trans == 0) {
+        for (int j = 0; j < n; ++j) {
+            const i64 id = hs_block_id(j, j, n);
+            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 0))) return rc;
+            if (j < n - 1 && (rc = hs_launch_col_update(h, L, x, b, j + 1, n - (j + 1), j, id, n, 0))) return rc;
+        }
+    } else {
+        for (int j = n - 1; j >= 0; --j) {
+            const i64 id = hs_block_id(j, j, n);
+            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 1))) return rc;
+            if (j > 0 && (rc = hs_launch_col_update(h, L, x, b, 0, j, j, id, n, 1))) return rc;
+        }
+    }
+    return HS_OK;
+}

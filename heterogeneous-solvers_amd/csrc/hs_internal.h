@@ -94,8 +94,10 @@ int hs_factor_trsm(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int b
 bool hs_trsv_fast_ok(int b);
 template <typename T>
 int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv);
+// phases: bit 0 forward (z = L^{-1} v), bit 1 backward (v = L^{-T} z)
 template <typename T>
-int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b);
+int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b,
+                          int phases = 3);
 int hs_launch_trsv_block(hs_handle_t h, const double* A, double* x, i64 N, int b, int blockRow, i64 blockID, int transposed);
 int hs_launch_col_update(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
                          i64 blockID, int n, int transposed);

@@ -210,14 +210,17 @@ int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv) {
 
 // v holds y on entry and x on exit; z (n*b, scratch) receives the forward
 // solution. Each diagonal step reads one vector and writes the other, so the
-// strips of one launch never read what another strip writes.
+// strips of one launch never read what another strip writes. phases: bit 0
+// runs the forward solve (z = L^{-1} v, v clobbered), bit 1 the backward
+// solve (v = L^{-T} z, z clobbered).
 template <typename T, int BS>
-static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N) {
+static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int phases) {
     const int n = (int)((N + BS - 1) / BS);
     const int strips = BS / 64;
     const double* Dup = Dinv;                      // D_j = L_jj^{-T}
     const double* Dlo = Dinv + (i64)n * BS * BS;   // L_jj^{-1}
     const dim3 blk(TS_THREADS);
+    if (phases & 1) {
     int rc = hs_memset_zero(h, z, sizeof(double) * (size_t)n * BS);
     if (rc) return rc;
     for (int j = 0; j < n; ++j) {
@@ -232,6 +235,8 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
             HS_LAUNCH_CHECK();
         }
     }
+    }
+    if (phases & 2) {
     for (int j = n - 1; j >= 0; --j) {
         // x_j = L_jj^{-T} z_j (entries past N keep their zero)
         gemv_rows_kernel<double, BS, 0><<<dim3((unsigned)strips), blk, 0, h->stream>>>(
@@ -243,22 +248,24 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
             HS_LAUNCH_CHECK();
         }
     }
+    }
     return HS_OK;
 }
 
 // Both solves in place on v (n*b entries), factor L (packed, T) with the
 // inverted diagonal blocks from hs_diag_inverse; z is an n*b scratch vector.
 template <typename T>
-int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b) {
+int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b,
+                          int phases) {
     switch (b) {
-        case 64: return tri_solve_bs<T, 64>(h, L, Dinv, v, z, N);
-        case 128: return tri_solve_bs<T, 128>(h, L, Dinv, v, z, N);
-        case 256: return tri_solve_bs<T, 256>(h, L, Dinv, v, z, N);
+        case 64: return tri_solve_bs<T, 64>(h, L, Dinv, v, z, N, phases);
+        case 128: return tri_solve_bs<T, 128>(h, L, Dinv, v, z, N, phases);
+        case 256: return tri_solve_bs<T, 256>(h, L, Dinv, v, z, N, phases);
         default: hs_set_error("fast triangular solve needs b in {64, 128, 256}"); return HS_ERR_UNSUPPORTED;
     }
 }
 
 template int hs_diag_inverse<double>(hs_handle_t, const double*, i64, int, double*);
 template int hs_diag_inverse<float>(hs_handle_t, const float*, i64, int, double*);
-template int hs_tri_solve_prepared<double>(hs_handle_t, const double*, const double*, double*, double*, i64, int);
-template int hs_tri_solve_prepared<float>(hs_handle_t, const float*, const double*, double*, double*, i64, int);
+template int hs_tri_solve_prepared<double>(hs_handle_t, const double*, const double*, double*, double*, i64, int, int);
+template int hs_tri_solve_prepared<float>(hs_handle_t, const float*, const double*, double*, double*, i64, int, int);

@@ -154,6 +154,12 @@ int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* ph
  * Replaces TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105). */
 int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b);
 
+/* One of the two substitutions, in place on x (device): trans = 0 solves
+ * L x' = x, trans = 1 solves L^T x' = x. Replaces the forward or the backward
+ * half of TriangularSystemSolver::solve (TriangularSystemSolver.cpp:31-56,
+ * 58-82). */
+int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b, int trans);
+
 /* Conjugate gradients with x0 = 0, true residual at iteration % 50 == 0,
  * stop when delta_new <= eps^2 delta_0 or after imax iterations; the loop
  * stays on the device. Outputs may be NULL.
@@ -209,6 +215,28 @@ typedef struct hs_refine_report_s {
  * HS_ERR_NOT_CONVERGED with the report filled in. */
 int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N, int b,
                     double tol, int max_ir, int max_pcg, hs_refine_report* report);
+
+/* ---- Gaussian-process likelihood (no reference counterpart; the forward
+ * pass is the reference's kernel matrix, MatrixGenerator.cpp:143-153, and its
+ * Cholesky solve, Cholesky.cpp:413-493, TriangularSystemSolver.cpp:15-105) -- */
+
+/* NLL = 1/2 y^T K^{-1} y + 1/2 log det K + N/2 log(2 pi) for the RBF kernel
+ * matrix of X (device, row-major N x d) with hyperparameters ell, sf2, sn2 and
+ * targets y (device, N). K (device, packed, n*(n+1)/2*b*b doubles) is
+ * overwritten by the factor L. matrix_bsz must be 64, 128 or 256 with
+ * matrix_bsz * d <= 4096. Returns 0, or the first bad pivot (>0) when K is
+ * not positive definite. */
+int hs_gp_nll_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, double ell, double sf2, double sn2,
+                  const double* y, double* K, double* nll);
+
+/* hs_gp_nll_f64 plus the gradient grad[0..2] (host) with respect to
+ * (log ell, log sf2, log sn2), by reverse-mode differentiation of the blocked
+ * Cholesky factorisation and solves. Kbar (device, packed like K) receives
+ * dNLL/dK in lower storage: an off-diagonal entry holds the derivative with
+ * respect to the (i,j) and (j,i) entries together, upper triangles of
+ * diagonal blocks are zero. */
+int hs_gp_nll_grad_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, double ell, double sf2, double sn2,
+                       const double* y, double* K, double* Kbar, double* nll, double* grad);
 
 #ifdef __cplusplus
 }

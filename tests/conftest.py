@@ -5,6 +5,14 @@ import pytest
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
+# torch ships its own HIP runtime under the same soname as the system one; it
+# must be loaded before libhs_amd.so pulls in the system copy, or torch's
+# device initialisation fails in a process that also uses the library.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 from hs_testlib import Oracle, load_hs  # noqa: E402
 
 

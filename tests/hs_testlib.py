@@ -4,6 +4,7 @@ synthetic GP workload generator written independently in numpy."""
 import ctypes as C
 import importlib.util
 import os
+import sys
 
 import numpy as np
 
@@ -20,10 +21,13 @@ _D = C.c_double
 def load_hs():
     """Import heterogeneous-solvers_amd/hs_amd.py (the directory name is not a
     Python identifier, so it is loaded by path)."""
+    if "hs_amd" in sys.modules:
+        return sys.modules["hs_amd"]
     path = os.path.join(ROOT, "heterogeneous-solvers_amd", "hs_amd.py")
     spec = importlib.util.spec_from_file_location("hs_amd", path)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
+    sys.modules["hs_amd"] = mod
     return mod
 
 
