@@ -8,9 +8,12 @@ on the N = 65536 GP system, inputs resident in HBM:
 value = (N^3/3) / step time in TFLOP/s (whole step, assembly and solves
 included). After the timed region, untimed checks and side measurements:
 backward error of the solution, per-phase kernel times of the factorisation
-(HIP events, for the roofline of the dominant kernel), the CG workload
-(N = 32768, b = 128) with its mat-vec bandwidth, and the CPU baseline (the
-oracle's restatement of the reference's CPU path on a bounded sample).
+(HIP events, for the roofline of the dominant kernel), the mixed-precision
+factor with FP64 refinement at the same backward-error target, the CG
+workload (N = 32768, b = 128) with its mat-vec bandwidth, the GP likelihood
+gradient (N = 16384, b = 128) checked against finite differences, and the CPU
+baseline (the oracle's restatement of the reference's CPU path on a bounded
+sample).
 
 Prints one JSON line.
 """
@@ -31,6 +34,13 @@ PKG = os.path.join(ROOT, "heterogeneous-solvers_amd")
 METRIC = "Cholesky N=65536 solve time+TFLOP/s: FP64 vs mixed+IR at equal residual; CG GB/s"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel (trailing_mfma_f64<128, 0>,
+# average over the 254 look-ahead rest-update launches at N = 65536, b = 256):
+# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors that make the
+# counters read 1 GiB for a copy kernel of 1 GiB read + 1 GiB written
+# (bench.py --copy-calibration). Summary and commands:
+# profiles/trailing_mfma_f64_cholesky65536_hbm_traffic.md
+TRAILING_TRAFFIC_BYTES = 16.28e9
 
 
 def _load(name, path):
@@ -222,6 +232,59 @@ def cg_workload(hs, dev, syn, args):
     }
 
 
+def adjoint_workload(hs, dev, syn, args):
+    """GP NLL and its gradient (log ell, log sf2, log sn2) at N = 16384, d = 8,
+    b = 128, ell = 1, sf2 = 1, sn2 = 1e-2: RBF assembly, FP64 factor, both
+    solves, the reverse pass through the blocked Cholesky and the contraction
+    with dK/dtheta. Checked against central differences (h = 1e-5)."""
+    import math
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["adjoint"]
+    n = -(-N // b)
+    X, y = syn.gp_inputs(seed, N, d)
+    dX, dy = dev.upload(X.reshape(-1)), dev.upload(y)
+    plen = packed_len(N, b)
+    dK, dKb = dev.alloc(plen * 8), dev.alloc(plen * 8)
+    nll, g = C.c_double(), (C.c_double * 3)()
+
+    def grad_call():
+        hs.call("hs_gp_nll_grad_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dy.p, dK.p, dKb.p, C.byref(nll), g)
+
+    def nll_at(th):
+        v = C.c_double()
+        hs.call("hs_gp_nll_f64", dev.h, dX.p, N, d, b, math.exp(th[0]), math.exp(th[1]), math.exp(th[2]), dy.p,
+                dK.p, C.byref(v))
+        return v.value
+
+    grad_call()
+    times = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        grad_call()
+        times.append((time.perf_counter() - t0) * 1e3)
+    t0 = time.perf_counter()
+    th = [math.log(ell), math.log(sf2), math.log(sn2)]
+    nll_at(th)
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    gv = [g[0], g[1], g[2]]
+    h = 1e-5
+    fd = []
+    for i in range(3):
+        tp, tm = list(th), list(th)
+        tp[i] += h
+        tm[i] -= h
+        fd.append((nll_at(tp) - nll_at(tm)) / (2 * h))
+    rel = max(abs(a - c) for a, c in zip(gv, fd)) / max(abs(a) for a in gv)
+    bwd_flops = sum(2.0 * m * m * b ** 3 + 4.0 * m * b ** 3 + 6.0 * b ** 3 for m in range(n))
+    ms = float(np.median(times))
+    dK.free()
+    dKb.free()
+    log("adjoint workload N=%d b=%d d=%d: nll+grad %.1f ms (nll alone %.1f ms), grad %s, fd %s, rel diff %.2e" %
+        (N, b, d, ms, fwd_ms, gv, fd, rel))
+    return {"N": N, "b": b, "d": d, "seed": seed, "ell": ell, "sf2": sf2, "sn2": sn2, "nll": nll.value,
+            "grad": gv, "fd_grad": fd, "fd_h": h, "fd_max_rel_diff": rel, "ms": ms, "nll_only_ms": fwd_ms,
+            "flops": N ** 3 / 3.0 + bwd_flops, "tflops": (N ** 3 / 3.0 + bwd_flops) / (ms / 1e3) / 1e12}
+
+
 def cpu_baseline(args):
     """The oracle (C restatement of the reference's CPU path, OpenMP) on a
     bounded sample of the same workload: the N = 8192 system of the same
@@ -256,8 +319,22 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a kernel of known HBM traffic (y = 2x over 2^27 doubles, 5 launches) to scale "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        hs = _load("hs_amd", os.path.join(PKG, "hs_amd.py"))
+        with hs.Device(0) as dev:
+            n = 1 << 27
+            x, y = dev.zeros(n), dev.zeros(n)
+            for _ in range(5):
+                hs.call("hs_vec_scale_f64", dev.h, x.p, 2.0, y.p, 256, 0, n // 256)
+            dev.sync()
+        print(json.dumps({"calibration": "hs_vec_scale_f64", "launches": 5, "read_bytes": n * 8, "write_bytes": n * 8}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would "
               "shard); run with --gpus 1", file=sys.stderr)
@@ -269,6 +346,7 @@ def main():
         ch = cholesky_workload(hs, dev, syn, args)
         mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args)
         cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
+        adj = None if args.no_adjoint else adjoint_workload(hs, dev, syn, args)
     if mixed:
         fp64_solve = ch["factor_ms"] + ch["solve_ms"]
         for v in mixed.values():
@@ -293,13 +371,16 @@ def main():
                    "signal_variance": ch["sf2"], "noise_variance": ch["sn2"], "parallelism": "1 GPU"},
         "roofline": {"kernel": "trailing_mfma_f64 (fused SYRK+GEMM trailing update)", "bound": "mfma",
                      "achieved": up["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": up["tflops"] / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": up["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": TRAILING_TRAFFIC_BYTES if ch["N"] == 65536 and ch["b"] == 256 else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "flops_per_launch": up["flops_per_launch"], "avg_launch_ms": up["avg_launch_ms"],
                      "launches": up["launches"]},
         "cpu_baseline": cpu,
         "cholesky": ch,
         "mixed": mixed,
         "cg": cg,
+        "adjoint": adj,
     }
     print(json.dumps(out))
 
