@@ -8,9 +8,10 @@
 //     colpart[ID] = A_IJ^T x_I         (b values, I > J only)
 //   pass 2, 64 rows of block row I per work-group, fixed order over J:
 //     y_i = sum_{J<=I} rowpart[ID(I,J)]_i + sum_{J>I} colpart[ID(J,I)]_i
-// Lanes read 16 contiguous bytes each, so one wave load covers 1 KB of a
-// block; row sums are finished by a recursive-halving shuffle reduction,
-// column sums stay in registers until a 4-wave LDS combine.
+// Sixteen lanes share a row, each reading 16 contiguous bytes per load, so a
+// wave load covers 256 bytes of each of 4 rows; row sums are finished by a
+// 16-lane recursive-halving shuffle reduction, column sums stay in registers
+// until an LDS combine.
 // The result is bit-reproducible run to run (no atomics).
 #include <algorithm>
 #include <climits>
@@ -63,6 +64,13 @@ __device__ inline void decode_block(i64 id, int n, int* I, int* J) {
 }
 
 // ABS: use |a_ij| (gives the row sums of |A| for x = 1, i.e. ||A||_inf)
+// Layout: 16 lanes share a row; lane l owns the columns 2(l%16) + 32 s + {0,1},
+// s < b/32, so one wave load covers 256 contiguous bytes of each of 4 rows
+// (quarter q = l/16) and a wave walks b/4 rows in groups of 4. A row sum is
+// then a 16-lane reduction (b/16 values over 16 lanes), which keeps the waves'
+// time between their streaming loads short; the column partials of the 16
+// (wave, quarter) pairs are added through LDS. Block data is read with
+// non-temporal loads: each element is used once per pass.
 template <int BS, bool ABS = false>
 __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const double* __restrict__ A,
                                                                       const double* __restrict__ x,
@@ -70,75 +78,65 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
                                                                       double* __restrict__ colpart, int n,
                                                                       const i64* __restrict__ stop, i64 it) {
     if (stop && it > *stop) return;
-    constexpr int LPR = BS / 2 < 64 ? BS / 2 : 64;  // lanes per row segment
-    constexpr int RPL = 64 / LPR;                    // rows per wave load
-    constexpr int NL = BS / (2 * LPR);               // double2 loads per row
-    constexpr int RPW = BS / 4;                      // rows per wave
-    constexpr int ITERS = RPW / RPL;                 // wave loads (per segment) per wave
-    constexpr int R = ITERS < 32 ? ITERS : 32;       // row partials reduced per group
-    constexpr int G = LPR;                           // lanes sharing one row
-    __shared__ double colred[4 * RPL][BS];
+    constexpr int NS = BS / 32;  // 16-byte loads per lane per row
+    constexpr int RW = BS / 4;   // rows per wave
+    constexpr int NG = RW / 4;   // 4-row groups per wave
+    constexpr int LPV = 16 / NG; // lanes per row total after the reduction
+    __shared__ double colred[16][BS];
 
     const i64 id = blockIdx.x;
     int I, J;
     decode_block(id, n, &I, &J);
     const bool diag = (I == J);
     const double* blk = A + id * (i64)BS * BS;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int sub = lane / LPR;   // which row of a wave load
-    const int sl = lane % LPR;    // lane within the row segment
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, c16 = lane & 15;
     const double* xJ = x + (i64)J * BS;
     const double* xI = x + (i64)I * BS;
 
-    double xj[2 * NL];
+    double xj[2 * NS], cacc[2 * NS];
 #pragma unroll
-    for (int s = 0; s < NL; ++s) {
-        xj[2 * s] = xJ[s * 2 * LPR + 2 * sl];
-        xj[2 * s + 1] = xJ[s * 2 * LPR + 2 * sl + 1];
+    for (int s = 0; s < NS; ++s) {
+        const double2 v = *reinterpret_cast<const double2*>(xJ + 2 * c16 + 32 * s);
+        xj[2 * s] = v.x;
+        xj[2 * s + 1] = v.y;
+        cacc[2 * s] = 0.0;
+        cacc[2 * s + 1] = 0.0;
     }
-    double cacc[2 * NL];
+    double p[NG];
 #pragma unroll
-    for (int s = 0; s < 2 * NL; ++s) cacc[s] = 0.0;
-
-    const int rowBase = wave * RPW;
+    for (int g = 0; g < NG; ++g) {
+        const int row = wave * RW + 4 * g + q;
+        const double* rp = blk + (i64)row * BS + 2 * c16;
+        const double xi = xI[row];
+        double acc = 0.0;
 #pragma unroll
-    for (int g0 = 0; g0 < ITERS; g0 += R) {
-        double p[R];
-#pragma unroll
-        for (int it2 = 0; it2 < R; ++it2) {
-            const int row = rowBase + (g0 + it2) * RPL + sub;
-            const double* rp = blk + (i64)row * BS + 2 * sl;
-            double acc = 0.0;
-            const double xi = xI[row];
-#pragma unroll
-            for (int s = 0; s < NL; ++s) {
-                double2 a = *reinterpret_cast<const double2*>(rp + s * 2 * LPR);
-                if (ABS) a = make_double2(fabs(a.x), fabs(a.y));
-                acc += a.x * xj[2 * s] + a.y * xj[2 * s + 1];
-                cacc[2 * s] += a.x * xi;
-                cacc[2 * s + 1] += a.y * xi;
+        for (int s = 0; s < NS; ++s) {
+            double a0 = __builtin_nontemporal_load(rp + 32 * s);
+            double a1 = __builtin_nontemporal_load(rp + 32 * s + 1);
+            if (ABS) {
+                a0 = fabs(a0);
+                a1 = fabs(a1);
             }
-            p[it2] = acc;
+            acc += a0 * xj[2 * s] + a1 * xj[2 * s + 1];
+            cacc[2 * s] += a0 * xi;
+            cacc[2 * s + 1] += a1 * xi;
         }
-        const double tot = halving_reduce<R, G>(p);
-        const int idx = sl >> (__builtin_ctz(G / R));
-        if ((sl & (G / R - 1)) == 0) {
-            const int row = rowBase + (g0 + idx) * RPL + sub;
-            rowpart[id * BS + row] = tot;
-        }
+        p[g] = acc;
     }
+    const double tot = halving_reduce<NG, 16>(p);
+    if ((c16 & (LPV - 1)) == 0) rowpart[id * BS + wave * RW + 4 * (c16 / LPV) + q] = tot;
     if (diag) return;  // diagonal blocks hold both triangles: row form only
 #pragma unroll
-    for (int s = 0; s < NL; ++s) {
-        colred[wave * RPL + sub][s * 2 * LPR + 2 * sl] = cacc[2 * s];
-        colred[wave * RPL + sub][s * 2 * LPR + 2 * sl + 1] = cacc[2 * s + 1];
+    for (int s = 0; s < NS; ++s) {
+        colred[wave * 4 + q][2 * c16 + 32 * s] = cacc[2 * s];
+        colred[wave * 4 + q][2 * c16 + 32 * s + 1] = cacc[2 * s + 1];
     }
     __syncthreads();
     for (int c = threadIdx.x; c < BS; c += SYMV_THREADS) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < 4 * RPL; ++w) s += colred[w][c];
+        for (int w = 0; w < 16; ++w) s += colred[w][c];
         colpart[id * BS + c] = s;
     }
 }
