@@ -107,6 +107,22 @@ def test_cholesky_and_solve_match_oracle(hs, dev, oracle, N, b):
     assert np.all(x[N:] == 0.0)
 
 
+@pytest.mark.parametrize("N,b,sn2", [(4096, 128, 4.0), (2048, 256, 16.0), (3000, 64, 8.0)])
+def test_solve_within_1e12_when_well_conditioned(hs, dev, oracle, N, b, sn2):
+    """The FP64 solution agrees with the oracle's (the reference algorithm on
+    the CPU) to 1e-12 relative where cond(K) * u leaves room for it."""
+    say("cholesky+solve 1e-12 N=%d b=%d sn2=%g" % (N, b, sn2))
+    X, A, rhs = rbf_system(oracle, N, 8, b, sn2=sn2)
+    xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
+    dA, dx = dev.upload(A), dev.upload(rhs)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+    x = dev.download(dx, len(rhs))
+    rel = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    say("  rel diff vs oracle %.3e" % rel)
+    assert rel <= 1e-12
+
+
 def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
     N, b = 256, 64
     K = np.eye(N)
@@ -131,6 +147,22 @@ def test_cg_matches_oracle(hs, dev, oracle, N, b, eps):
     K = packed_to_dense(A, N, b)
     assert np.linalg.norm(K @ x[:N] - rhs[:N]) <= 3 * eps * np.linalg.norm(rhs[:N])
     assert df.value <= eps * eps * d0.value
+
+
+@pytest.mark.parametrize("N,b,sn2", [(4096, 128, 4.0), (2500, 64, 1.0), (8192, 256, 16.0)])
+def test_cg_iterations_within_one(hs, dev, oracle, N, b, sn2):
+    """Where the convergence history is not dominated by rounding (moderate
+    condition number), the iteration count matches the oracle's to +-1."""
+    say("cg iterations N=%d b=%d sn2=%g" % (N, b, sn2))
+    _, A, rhs = rbf_system(oracle, N, 8, b, sn2=sn2)
+    xref, itref, _ = oracle.cg(A, rhs, N, b, 1e-10)
+    dA, drhs, dx = dev.upload(A), dev.upload(rhs), dev.zeros(len(rhs))
+    it = C.c_int64()
+    hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100000, C.byref(it), None, None)
+    say("  iterations gpu %d oracle %d" % (it.value, itref))
+    assert abs(it.value - itref) <= 1
+    x = dev.download(dx, len(rhs))
+    assert np.linalg.norm(x - xref) <= 1e-9 * np.linalg.norm(xref)
 
 
 def test_cg_imax_and_zero_rhs(hs, dev, oracle):
