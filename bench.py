@@ -35,7 +35,7 @@ METRIC = "Cholesky N=65536 solve time+TFLOP/s: FP64 vs mixed+IR at equal residua
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (trailing_mfma_f64<128, 0>,
-# average over the 254 look-ahead rest-update launches at N = 65536, b = 256):
+# average over the look-ahead rest-update launches at N = 65536, b = 256):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors that make the
 # counters read 1 GiB for a copy kernel of 1 GiB read + 1 GiB written
 # (bench.py --copy-calibration). Summary and commands:
@@ -57,6 +57,12 @@ def log(*a):
 def packed_len(N, b):
     n = -(-N // b)
     return n * (n + 1) // 2 * b * b
+
+
+def rest_update_work(hs, N, b):
+    cnt, fl = C.c_int64(), C.c_double()
+    hs.call("hs_cholesky_schedule_f64", N, b, C.byref(cnt), C.byref(fl))
+    return cnt.value, fl.value
 
 
 def cholesky_workload(hs, dev, syn, args):
@@ -120,12 +126,11 @@ def cholesky_workload(hs, dev, syn, args):
     flops = N ** 3 / 3.0
     ms = float(np.median(step_ms))
     nl = int(launches[2])
-    # algorithmic flops of the timed trailing-update launches: step k updates
-    # m = n-k-1 block rows; its full update is m^2 b^3 flops (m SYRK blocks at
-    # b^3 + m(m-1)/2 GEMM blocks at 2 b^3). With the look-ahead, the main-stream
-    # launch of step k leaves out block column k+1 and covers (m-1)^2 b^3.
-    top = n - 1 if nl == n - 1 else n - 2
-    upd_flops = sum(j * j for j in range(1, top + 1)) * float(b) ** 3
+    # algorithmic flops of the timed trailing-update launches (the look-ahead
+    # rest updates): b^3 per diagonal and 2 b^3 per off-diagonal block and panel
+    sl, upd_flops = rest_update_work(hs, N, b)
+    if sl != nl:
+        raise RuntimeError("schedule reports %d rest-update launches, timer saw %d" % (sl, nl))
     achieved = upd_flops / (ph[2] / 1e3) / 1e12
     return {
         "N": N, "b": b, "d": d, "seed": seed, "ell": ell, "sf2": sf2, "sn2": sn2,
@@ -180,8 +185,7 @@ def mixed_workload(hs, dev, syn, args):
         fac = float(np.median([r[0] for r in runs]))
         ref = float(np.median([r[1] for r in runs]))
         ph = np.mean([r[2] for r in runs], axis=0)
-        top = n - 2 if int(launches[2]) == n - 2 else n - 1
-        upd_flops = sum(j * j for j in range(1, top + 1)) * float(b) ** 3
+        _, upd_flops = rest_update_work(hs, N, b)
         out[name] = {"factor_ms": fac, "refine_ms": ref, "solve_ms": fac + ref, "report": runs[-1][3],
                      "update_kernel_ms": float(ph[2]), "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
                      "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}
@@ -210,6 +214,8 @@ def cg_workload(hs, dev, syn, args):
         hs.call("hs_symv_packed_f64", dev.h, dA.p, drhs.p, dq.p, N, b)
     timer.stop()
     symv_ms = timer.ms() / reps
+    passes = (C.c_double * 2)()
+    hs.call("hs_symv_packed_f64_timed", dev.h, dA.p, drhs.p, dq.p, N, b, reps, passes)
     it, df, d0 = C.c_int64(), C.c_double(), C.c_double()
     timer.start()
     hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100000, C.byref(it), C.byref(df), C.byref(d0))
@@ -222,11 +228,15 @@ def cg_workload(hs, dev, syn, args):
     rel_res = float(np.linalg.norm(rhs - dev.download(dq, n * b)) / np.linalg.norm(rhs))
     dA.free()
     bytes_symv = plen * 8 + 16 * n * b
-    log("cg workload N=%d b=%d: %d iterations in %.1f ms, symv %.3f ms (%.0f GB/s)" %
-        (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6))
+    log("cg workload N=%d b=%d: %d iterations in %.1f ms, symv %.3f ms (%.0f GB/s); block pass %.3f ms "
+        "(%.0f GB/s), reduction pass %.3f ms" % (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6,
+                                                 passes[0], bytes_symv / passes[0] / 1e6, passes[1]))
     return {
         "N": N, "b": b, "seed": seed, "eps": 1e-10, "iterations": iters, "ms": cg_ms,
         "symv_count": nsymv, "symv_ms": symv_ms, "symv_gbs": bytes_symv / (symv_ms / 1e3) / 1e9,
+        "symv_block_pass_ms": passes[0], "symv_reduce_pass_ms": passes[1],
+        "symv_block_pass_gbs": bytes_symv / (passes[0] / 1e3) / 1e9,
+        "symv_block_pass_hbm_frac": bytes_symv / (passes[0] / 1e3) / 1e9 / HBM_PEAK_GBS,
         "cg_gbs": nsymv * bytes_symv / (cg_ms / 1e3) / 1e9, "rel_residual_true": rel_res,
         "delta_ratio": df.value / d0.value if d0.value else 0.0,
     }

@@ -161,11 +161,28 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_finish_kernel(const double*
     const i64 i = (i64)blockIdx.x * FIN_ROWS + lane;
     const int I = (int)(i / b), ii = (int)(i % b);
     const int j0 = (int)((i64)wave * n / 4), j1 = (int)((i64)(wave + 1) * n / 4);
+    // 8 loads in flight per step; the sum itself stays in J order
     double s = 0.0;
     const int jr = min(j1, I + 1);
-    for (int J = j0; J < jr; ++J) s += rowpart[hs_block_id(I, J, n) * b + ii];
+    int J = j0;
+    for (; J + 8 <= jr; J += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = rowpart[hs_block_id(I, J + u, n) * b + ii];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; J < jr; ++J) s += rowpart[hs_block_id(I, J, n) * b + ii];
     const i64 base = hs_block_id(I, I, n);
-    for (int J = max(j0, I + 1); J < j1; ++J) s += colpart[(base + (J - I)) * b + ii];
+    J = max(j0, I + 1);
+    for (; J + 8 <= j1; J += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = colpart[(base + (J + u - I)) * b + ii];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; J < j1; ++J) s += colpart[(base + (J - I)) * b + ii];
     red[wave][lane] = s;
     __syncthreads();
     if (wave != 0) return;
@@ -378,6 +395,53 @@ extern "C" int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* 
     symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(w.rowpart, w.colpart, n, b, y,
                                                                                    nullptr, nullptr, nullptr, 0);
     HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+// hs_symv_packed_f64 repeated `reps` times with HIP events around each pass on
+// the handle's stream: pass_ms[0] = average duration of the read-once block
+// pass, pass_ms[1] = of the fixed-order row reduction.
+extern "C" int hs_symv_packed_f64_timed(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b,
+                                        int reps, double* pass_ms) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (!x) return -3;
+    if (!y) return -4;
+    if (N <= 0) return -5;
+    if (!symv_fast_ok(b)) {
+        hs_set_error("hs_symv_packed_f64_timed: block size must be 64, 128 or 256");
+        return -6;
+    }
+    if (reps <= 0) return -7;
+    if (!pass_ms) return -8;
+    const int n = (int)((N + b - 1) / b);
+    SymvWork w;
+    int rc = symv_workspace(h, n, b, false, &w);
+    if (rc) return rc;
+    hipEvent_t ev[3];
+    for (int e = 0; e < 3; ++e) HS_HIP(hipEventCreate(&ev[e]));
+    double t0 = 0.0, t1 = 0.0;
+    const i64 G = (i64)n * b / FIN_ROWS;
+    for (int r = 0; r < reps && rc == HS_OK; ++r) {
+        HS_HIP(hipEventRecord(ev[0], h->stream));
+        rc = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, nullptr, 0);
+        if (rc) break;
+        HS_HIP(hipEventRecord(ev[1], h->stream));
+        symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(
+            w.rowpart, w.colpart, n, b, y, nullptr, nullptr, nullptr, 0);
+        HS_LAUNCH_CHECK();
+        HS_HIP(hipEventRecord(ev[2], h->stream));
+        HS_HIP(hipEventSynchronize(ev[2]));
+        float a = 0.f, c = 0.f;
+        HS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HS_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
+        t0 += a;
+        t1 += c;
+    }
+    for (int e = 0; e < 3; ++e) hipEventDestroy(ev[e]);
+    if (rc) return rc;
+    pass_ms[0] = t0 / reps;
+    pass_ms[1] = t1 / reps;
     return HS_OK;
 }
 

@@ -25,8 +25,8 @@ namespace {
 constexpr int KT = 16;
 constexpr int THREADS = 256;
 
-// KIND only separates the look-ahead column launches (KIND 1) from the rest
-// in profiles; the code is the same.
+// KIND only separates the look-ahead column launches (KIND 1, modes 5 and 6)
+// from the rest in profiles; the code is the same.
 template <int TILE, int KIND>
 __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict__ A, int b, int n, PairMap pm) {
     constexpr int WT = TILE / 2;   // wave tile edge
@@ -69,12 +69,15 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 #pragma unroll
         for (int j = 0; j < MT; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    const int nk = b / KT;
+    // np panels: panel p + 1 of a block row starts n - k - 1 blocks after
+    // panel p (column-major block order), the same step for every row
+    const int nk = b / KT, nkt = nk * pm.np;
+    const i64 pstep = (i64)(n - pm.k - 1) * bb;
     auto gload = [&](int kt) {
 #pragma unroll
         for (int v = 0; v < LPT / 2; ++v) {
-            ra[v] = *reinterpret_cast<const double2*>(pa + kt * KT + 2 * v);
-            rb[v] = *reinterpret_cast<const double2*>(pb + kt * KT + 2 * v);
+            ra[v] = *reinterpret_cast<const double2*>(pa + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 2 * v);
+            rb[v] = *reinterpret_cast<const double2*>(pb + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 2 * v);
         }
     };
     auto sstore = [&](int buf) {
@@ -93,10 +96,10 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
     // in the other buffer and stage kt+2 is in flight from global memory
     gload(0);
     sstore(0);
-    if (nk > 1) gload(1);
+    if (nkt > 1) gload(1);
     __syncthreads();
     const int lr = lane & 15, lq = lane >> 4;
-    for (int kt = 0; kt < nk; ++kt) {
+    for (int kt = 0; kt < nkt; ++kt) {
         const double* as = As + (kt & 1) * STAGE;
         const double* bs = Bs + (kt & 1) * STAGE;
 #pragma unroll
@@ -112,9 +115,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
                 for (int j = 0; j < MT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (kt + 1 < nk) {
+        if (kt + 1 < nkt) {
             sstore((kt + 1) & 1);
-            if (kt + 2 < nk) gload(kt + 2);
+            if (kt + 2 < nkt) gload(kt + 2);
         }
         __syncthreads();
     }
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 // GEMM has the same constraint, MatrixMatrixOperations.cpp:657-659), else the
 // element-wise kernel.
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
-                          int mode) {
+                          int mode, int np) {
     if (rowCount <= 0) return HS_OK;
     if (b % 64 != 0) {
         if (mode > 3) {
@@ -176,7 +179,7 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
     const int tile = (b % 128 == 0) ? 128 : 64;
     const i64 tilesPerBlock = (i64)(b / tile) * (b / tile);
     PairMap pm;
-    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, tilesPerBlock, &pm);
+    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, np, tilesPerBlock, &pm);
     if (grid <= 0) return HS_OK;
     if (grid > 0x7fffffffLL) {
         hs_set_error("trailing update grid too large");
@@ -184,10 +187,10 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
     }
     const dim3 g((unsigned)grid), t(THREADS);
     if (tile == 128) {
-        if (mode == 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode >= 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
     } else {
-        if (mode == 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode >= 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f64<64, 0><<<g, t, 0, stream>>>(A, b, n, pm);
     }
     HS_LAUNCH_CHECK();
@@ -195,7 +198,7 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
 }
 
 int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode) {
-    return hs_launch_trailing_on(h, h->stream, A, b, k, rowStart, rowCount, n, mode);
+    return hs_launch_trailing_on(h, h->stream, A, b, k, rowStart, rowCount, n, mode, 1);
 }
 
 // Look-ahead FP64 factorisation (schedule in hs_factor.h).
@@ -207,8 +210,8 @@ static int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer
     auto trsm = [&](int k, int start, int count) {
         return hs_factor_trsm<double>(h, A, b, hs_block_id(k, k, n), k, start, count);
     };
-    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
-        return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode);
+    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
+        return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode, np);
     };
     return hs_lookahead_factor(h, n, pt, potrf, trsm, trail);
 }
@@ -353,5 +356,16 @@ extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int
             if (j > 0 && (rc = hs_launch_col_update(h, L, x, b, 0, j, j, id, n, 1))) return rc;
         }
     }
+    return HS_OK;
+}
+
+// Work of the rest-update launches of the look-ahead schedule the factorisations
+// use for N and b (b a multiple of 64): their count and algorithmic flops.
+extern "C" int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops) {
+    if (N <= 0) return -1;
+    if (b <= 0 || b % 64 != 0) return -2;
+    if (!rest_launches) return -3;
+    if (!rest_flops) return -4;
+    hs_rest_update_work((int)((N + b - 1) / b), b, rest_launches, rest_flops);
     return HS_OK;
 }

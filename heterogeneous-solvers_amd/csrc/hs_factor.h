@@ -12,15 +12,18 @@
 
 // Which (I, J) blocks one trailing-update launch of step k covers; the
 // launch's work-group q (one per block, times the tiles of a block) decodes
-// its pair with pair_of:
+// its pair with pair_of. A launch applies np consecutive panels k .. k+np-1:
+// A_IJ -= sum_p L_I,k+p L_J,k+p^T (np = 2 on the two-panel schedule).
 //  mode 1: diagonal blocks I = J = rowStart + q             (SYRK entry)
 //  mode 2: rows k+2.., J in [k+1, I-1], offset pairOffset    (GEMM entry)
 //  mode 3: rows k+1.., J in [k+1, I],   offset pairOffset    (whole step)
-//  mode 4: rows k+2.., J in [k+2, I]                         (look-ahead rest)
-//  mode 5: J = k+1, I = k+1+q                                (look-ahead panel column)
+//  mode 4: rows and columns from rs = k + 2 np, J <= I       (look-ahead rest)
+//  mode 5: J = rs = k+1, I = rs + q                          (look-ahead panel column)
+//  mode 6: J in {rs, rs+1} with rs = k + np, I >= J          (two look-ahead columns)
 struct PairMap {
     int k, rowStart, mode;
     i64 pairOffset;
+    int n = 0, np = 1;
     // band order (modes 3 and 4 over a whole triangle): m rows, bands of
     // `band` block rows walked column by column; m == 0 keeps pair_of's order
     int m = 0, band = 4;
@@ -37,15 +40,26 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
         return;
     }
     if (pm.mode == 5) {
-        *I = pm.k + 1 + (int)q;
-        *J = pm.k + 1;
+        *I = pm.rowStart + (int)q;
+        *J = pm.rowStart;
+        return;
+    }
+    if (pm.mode == 6) {
+        const int m1 = pm.n - pm.rowStart;
+        if (q < m1) {
+            *I = pm.rowStart + (int)q;
+            *J = pm.rowStart;
+        } else {
+            *I = pm.rowStart + 1 + (int)(q - m1);
+            *J = pm.rowStart + 1;
+        }
         return;
     }
     int r, c;
     if (pm.mode == 4) {
         hs_tri_decode(q, &r, &c);
-        *I = pm.k + 2 + r;
-        *J = pm.k + 2 + c;
+        *I = pm.rowStart + r;
+        *J = pm.rowStart + c;
         return;
     }
     hs_tri_decode(q + pm.pairOffset, &r, &c);
@@ -112,18 +126,22 @@ __device__ inline bool tile_of(const PairMap& pm, int tilesPerBlock, int* I, int
 }
 
 // Number of block pairs and the pair offset of a launch (see PairMap).
-inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, i64* pairs, i64* offset, int* rs) {
+inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, int np, i64* pairs, i64* offset,
+                          int* rs) {
     *offset = 0;
     *rs = rowStart;
     if (mode == 1) {
         *pairs = rowCount;
     } else if (mode == 4) {
-        const i64 m = n - k - 2;
+        *rs = k + 2 * np;
+        const i64 m = n - *rs;
         *pairs = m > 0 ? m * (m + 1) / 2 : 0;
-        *rs = k + 2;
     } else if (mode == 5) {
-        *pairs = n - k - 1;
         *rs = k + 1;
+        *pairs = std::max(0, n - *rs);
+    } else if (mode == 6) {
+        *rs = k + np;
+        *pairs = std::max(0, n - *rs) + std::max(0, n - *rs - 1);
     } else if (mode == 2) {
         const int firstRow = std::max(rowStart, k + 2);
         const int lastRow = rowStart + rowCount;
@@ -158,16 +176,18 @@ inline void hs_trail_order(int* bits, int* band) {
 
 // Fills the pair map of a trailing-update launch with tilesPerBlock work-
 // groups per block pair; returns the grid size (0: nothing to do).
-inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, i64 tilesPerBlock, PairMap* pm) {
+inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int np, i64 tilesPerBlock, PairMap* pm) {
     i64 pairs, off;
     int rs;
-    hs_pair_range(k, rowStart, rowCount, n, mode, &pairs, &off, &rs);
+    hs_pair_range(k, rowStart, rowCount, n, mode, np, &pairs, &off, &rs);
     *pm = PairMap{k, rs, mode, off};
+    pm->n = n;
+    pm->np = np;
     if (pairs <= 0) return 0;
     int bits, band;
     hs_trail_order(&bits, &band);
     if ((bits & 1) && (mode == 4 || (mode == 3 && off == 0))) {
-        pm->m = (mode == 4) ? n - k - 2 : rowCount;
+        pm->m = (mode == 4) ? n - rs : rowCount;
         pm->band = band;
     }
     const i64 total = pairs * tilesPerBlock;
@@ -230,49 +250,87 @@ struct PhaseTimer {
     }
 };
 
-// Right-looking factorisation with a look-ahead of one panel on two streams
-// of the same priority. Stream S2 factors panels: it applies step k's update
-// to block column k+1 only (trail mode 5), then POTRF and TRSM of panel k+1.
-// Stream S1 applies step k's update to the rest of the trailing triangle
-// (mode 4, J >= k+2) as soon as panel k is final, so the panel work of k+1
-// runs underneath it. S1's update k waits for panel k (event P[k]); S2's
-// column update of step k+1 waits for S1's update k, which also wrote block
-// column k+2.
+// Panels per step of the look-ahead schedule: 2 (trailing updates of depth
+// 2b) unless HS_PANELS=1, and 1 for fewer than 4 block columns.
+inline int hs_schedule_panels(int n) {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("HS_PANELS");
+        v = (e && std::atoi(e) == 1) ? 1 : 2;
+    }
+    return n >= 4 ? v : 1;
+}
+
+// Launches and algorithmic flops of the rest-update launches (stream S1) of
+// the schedule for n block columns of size b: step k (k = 0, P, 2P, ...)
+// updates the m = n - k - 2P block rows from row k + 2P with P panels, i.e.
+// P b^3 m^2 flops (2 P b^3 per off-diagonal block, P b^3 per diagonal block,
+// whose upper half is not needed).
+inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops) {
+    const int P = hs_schedule_panels(n);
+    int64_t cnt = 0;
+    double f = 0.0;
+    for (int k = 0; k + P < n; k += P) {
+        const double m = n - k - 2 * P;
+        if (m <= 0) continue;
+        ++cnt;
+        f += (double)P * b * b * (double)b * m * m;
+    }
+    *launches = cnt;
+    *flops = f;
+}
+
+// Right-looking factorisation with a look-ahead of one step on two streams of
+// the same priority; a step applies P panels (P = hs_schedule_panels(n)).
+// Stream S2 factors panels: the step's columns k+P .. k+2P-1 get the update
+// by panels k .. k+P-1 (trail mode 5 for P = 1, mode 6 for P = 2), then for
+// each of the next P panels POTRF and TRSM, the second panel's column first
+// updated by the first one (mode 5). Stream S1 applies the step's panels to
+// the rest of the trailing triangle (mode 4, rows and columns >= k + 2P) as
+// soon as they are final, so the next step's panel work runs underneath it.
+// S1's update of step s waits for the panels of step s (event P[s]); S2's
+// column update of step s+1 waits for S1's update of step s-1, which wrote
+// those columns.
 //   potrf(k), trsm(k, blockStart, blockCount): launch on h->stream
-//   trail(stream, k, mode, rowStart, rowCount): launch on `stream`
+//   trail(stream, k, mode, rowStart, rowCount, np): launch on `stream`
 template <class Potrf, class Trsm, class Trail>
 int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm trsm, Trail trail) {
+    const int P = hs_schedule_panels(n);
     int rc = hs_ensure_events(h, 2 * (size_t)n + 2);
     if (rc) return rc;
-    hipEvent_t* evP = h->events.data();          // panel k final (S2)
-    hipEvent_t* evU = h->events.data() + n + 1;  // S1 update k done
+    hipEvent_t* evP = h->events.data();          // panels of step s final (S2)
+    hipEvent_t* evU = h->events.data() + n + 1;  // S1 update of step s done
     hipStream_t s1 = h->stream, s2 = h->stream2;
     // S2 must see everything already queued on S1 (e.g. the input assembly)
     HS_HIP(hipEventRecord(evU[n], s1));
     HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
-    {
+    // POTRF and TRSM of panels k0 .. k0+P-1 on S2
+    auto chain = [&](int k0) -> int {
         OnStream os(h, s2);
-        if ((rc = pt.begin(s2, 0)) || (rc = potrf(0)) || (rc = pt.end(s2, 0))) return rc;
-        if (n > 1 && ((rc = pt.begin(s2, 1)) || (rc = trsm(0, 1, n - 1)) || (rc = pt.end(s2, 1)))) return rc;
-    }
+        int r;
+        for (int k = k0; k < std::min(n, k0 + P); ++k) {
+            if (k > k0 && (r = trail(s2, k - 1, 5, k, n - k, 1))) return r;
+            if ((r = pt.begin(s2, 0)) || (r = potrf(k)) || (r = pt.end(s2, 0))) return r;
+            if (k + 1 < n && ((r = pt.begin(s2, 1)) || (r = trsm(k, k + 1, n - k - 1)) || (r = pt.end(s2, 1))))
+                return r;
+        }
+        return HS_OK;
+    };
+    if ((rc = chain(0))) return rc;
     HS_HIP(hipEventRecord(evP[0], s2));
-    for (int k = 0; k + 1 < n; ++k) {
-        const int m = n - k - 1;
-        if (k >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[k - 1], 0));
-        if ((rc = trail(s2, k, 5, k + 1, m))) return rc;
-        {
-            OnStream os(h, s2);
-            if ((rc = pt.begin(s2, 0)) || (rc = potrf(k + 1)) || (rc = pt.end(s2, 0))) return rc;
-            if (m > 1 && ((rc = pt.begin(s2, 1)) || (rc = trsm(k + 1, k + 2, m - 1)) || (rc = pt.end(s2, 1))))
+    int step = 0;
+    for (int k = 0; k + P < n; k += P, ++step) {
+        if (step >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[step - 1], 0));
+        if ((rc = trail(s2, k, P == 1 ? 5 : 6, k + P, n - k - P, P))) return rc;
+        if ((rc = chain(k + P))) return rc;
+        HS_HIP(hipEventRecord(evP[step + 1], s2));
+        HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
+        if (k + 2 * P < n) {
+            if ((rc = pt.begin(s1, 2)) || (rc = trail(s1, k, 4, k + 2 * P, n - k - 2 * P, P)) || (rc = pt.end(s1, 2)))
                 return rc;
         }
-        HS_HIP(hipEventRecord(evP[k + 1], s2));
-        HS_HIP(hipStreamWaitEvent(s1, evP[k], 0));
-        if (m > 1) {
-            if ((rc = pt.begin(s1, 2)) || (rc = trail(s1, k, 4, k + 2, m - 1)) || (rc = pt.end(s1, 2))) return rc;
-        }
-        HS_HIP(hipEventRecord(evU[k], s1));
+        HS_HIP(hipEventRecord(evU[step], s1));
     }
-    HS_HIP(hipStreamWaitEvent(s1, evP[n - 1], 0));
+    HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
     return HS_OK;
 }

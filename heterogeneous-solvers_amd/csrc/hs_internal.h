@@ -69,10 +69,11 @@ int hs_launch_trsm(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, i
 // trailing update over rows [rowStart, rowStart+rowCount) of step k;
 // mode 1 = diagonal blocks only (SYRK), 2 = off-diagonal only (GEMM), 3 = both
 int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int rowCount, int n, int mode);
-// same on an explicit stream; modes 4 (J >= k+2 rows k+2..n-1) and 5 (block
-// column k+1, rows k+1..n-1) split step k for the look-ahead driver
+// same on an explicit stream with np panels k .. k+np-1; modes 4 (the
+// triangle from row k + 2 np), 5 (block column k+1) and 6 (block columns
+// k+np, k+np+1) split a step for the look-ahead driver
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
-                          int mode);
+                          int mode, int np);
 // panel kernels for b % 32 == 0 (POTRF) / b % 16 == 0 (TRSM), b <= 256;
 // W = the 16x16 diagonal sub-block inverses of the factor (POTRF writes them
 // when W is non-null; a TRSM given W == nullptr computes them into trsm_w)

@@ -115,12 +115,15 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
     for (int i = 0; i < MT; ++i)
 #pragma unroll
         for (int j = 0; j < MT; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int nk = b / KT;
+    // np panels: panel p + 1 of a block row starts n - k - 1 blocks after
+    // panel p (column-major block order), the same step for every row
+    const int nk = b / KT, nkt = nk * pm.np;
+    const i64 pstep = (i64)(n - pm.k - 1) * bb;
     auto gload = [&](int kt) {
 #pragma unroll
         for (int v = 0; v < LPT / 4; ++v) {
-            ra[v] = *reinterpret_cast<const float4*>(pa + kt * KT + 4 * v);
-            rb[v] = *reinterpret_cast<const float4*>(pb + kt * KT + 4 * v);
+            ra[v] = *reinterpret_cast<const float4*>(pa + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 4 * v);
+            rb[v] = *reinterpret_cast<const float4*>(pb + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 4 * v);
         }
     };
     auto sstore = [&](int buf) {
@@ -140,10 +143,10 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
     };
     gload(0);
     sstore(0);
-    if (nk > 1) gload(1);
+    if (nkt > 1) gload(1);
     __syncthreads();
     const int lr = lane & 15, lq = lane >> 4;
-    for (int kt = 0; kt < nk; ++kt) {
+    for (int kt = 0; kt < nkt; ++kt) {
         const float* as = As + (kt & 1) * STAGE;
         const float* bs = Bs + (kt & 1) * STAGE;
 #pragma unroll
@@ -159,9 +162,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
                 for (int j = 0; j < MT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (kt + 1 < nk) {
+        if (kt + 1 < nkt) {
             sstore((kt + 1) & 1);
-            if (kt + 2 < nk) gload(kt + 2);
+            if (kt + 2 < nkt) gload(kt + 2);
         }
         __syncthreads();
     }
@@ -211,12 +214,15 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     for (int i = 0; i < MT; ++i)
 #pragma unroll
         for (int j = 0; j < MT; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int nk = b / KT;
+    // np panels: panel p + 1 of a block row starts n - k - 1 blocks after
+    // panel p (column-major block order), the same step for every row
+    const int nk = b / KT, nkt = nk * pm.np;
+    const i64 pstep = (i64)(n - pm.k - 1) * bb;
     auto gload = [&](int kt) {
 #pragma unroll
         for (int v = 0; v < LPT / 4; ++v) {
-            ra[v] = *reinterpret_cast<const float4*>(pa + kt * KT + 4 * v);
-            rb[v] = *reinterpret_cast<const float4*>(pb + kt * KT + 4 * v);
+            ra[v] = *reinterpret_cast<const float4*>(pa + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 4 * v);
+            rb[v] = *reinterpret_cast<const float4*>(pb + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 4 * v);
         }
     };
     auto split_store = [&](const float4* r, unsigned short* hiPlane, unsigned short* loPlane) {
@@ -255,10 +261,10 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     };
     gload(0);
     sstore(0);
-    if (nk > 1) gload(1);
+    if (nkt > 1) gload(1);
     __syncthreads();
     const int lr = lane & 15, lq = lane >> 4;
-    for (int kt = 0; kt < nk; ++kt) {
+    for (int kt = 0; kt < nkt; ++kt) {
         const unsigned short* s = lds + (kt & 1) * STAGE;
         bf8 ah[MT], al[MT], bh[MT], bl[MT];
 #pragma unroll
@@ -281,9 +287,9 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
             }
-        if (kt + 1 < nk) {
+        if (kt + 1 < nkt) {
             sstore((kt + 1) & 1);
-            if (kt + 2 < nk) gload(kt + 2);
+            if (kt + 2 < nkt) gload(kt + 2);
         }
         __syncthreads();
     }
@@ -293,7 +299,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
 }  // namespace
 
 int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
-                             int n, int mode, int variant) {
+                             int n, int mode, int variant, int np) {
     if (rowCount <= 0) return HS_OK;
     if (b % 128 != 0) {
         hs_set_error("mixed-precision trailing update needs b % 128 == 0");
@@ -302,7 +308,7 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     constexpr int TILE = 128;
     const i64 tpb = (i64)(b / TILE) * (b / TILE);
     PairMap pm;
-    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, tpb, &pm);
+    const i64 grid = hs_pair_map(k, rowStart, rowCount, n, mode, np, tpb, &pm);
     if (grid <= 0) return HS_OK;
     if (grid > 0x7fffffffLL) {
         hs_set_error("trailing update grid too large");
@@ -310,10 +316,10 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     }
     const dim3 g((unsigned)grid), t(THREADS);
     if (variant == HS_MIXED_BF16X3) {
-        if (mode == 5) trailing_mfma_bf16x3<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode >= 5) trailing_mfma_bf16x3<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_bf16x3<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
     } else {
-        if (mode == 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode >= 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f32<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
     }
     HS_LAUNCH_CHECK();
@@ -345,8 +351,8 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     auto trsm = [&](int k, int start, int count) {
         return hs_factor_trsm<float>(h, L32, b, hs_block_id(k, k, n), k, start, count);
     };
-    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount) {
-        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant);
+    auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
+        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np);
     };
     if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
         hipStreamSynchronize(h->stream2);

@@ -113,6 +113,12 @@ int hs_symv_block_f64(hs_handle_t h, const double* A, const double* x, double* r
  * applying it to both triangles; deterministic (fixed-order) reduction.
  * Replaces matrixVectorBlock_GPU at share 1.0 (CG.cpp:364-369). */
 int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b);
+/* hs_symv_packed_f64 repeated reps times with HIP events around each of its
+ * two passes on the handle's stream; pass_ms[0] / pass_ms[1] receive the
+ * average durations of the read-once block pass and of the row reduction.
+ * Measurement entry; no reference counterpart. */
+int hs_symv_packed_f64_timed(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b, int reps,
+                             double* pass_ms);
 
 /* x_J <- L_JJ^{-1} x_J (transposed: L_JJ^{-T} x_J), entries >= N untouched.
  * Replaces MatrixVectorOperations::triangularSolveBlockVector (:282-335). */
@@ -149,6 +155,12 @@ int hs_cholesky_f64(hs_handle_t h, double* A, int64_t N, int b);
  * launches[0..2] their counts. Replaces the per-step kernel timing the
  * reference collects into MetricsTracker (Cholesky.cpp:261-359). */
 int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* phase_ms, int64_t* launches);
+
+/* The rest-update launches of the look-ahead schedule hs_cholesky_f64 and
+ * hs_cholesky_mixed use for N and b (b a multiple of 64): their number and
+ * their algorithmic flops (b^3 per diagonal block and 2 b^3 per off-diagonal
+ * block and panel). Measurement helper; no reference counterpart. */
+int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops);
 
 /* Forward L z = x then backward L^T y = z, in place on x (device).
  * Replaces TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105). */
