@@ -71,7 +71,7 @@ __device__ inline void decode_block(i64 id, int n, int* I, int* J) {
 // time between their streaming loads short; the column partials of the 16
 // (wave, quarter) pairs are added through LDS. Block data is read with
 // non-temporal loads: each element is used once per pass.
-template <int BS, bool ABS = false>
+template <int BS, bool ABS = false, int GB = 4, int ROLL = 1>
 __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const double* __restrict__ A,
                                                                       const double* __restrict__ x,
                                                                       double* __restrict__ rowpart,
@@ -81,7 +81,6 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
     constexpr int NS = BS / 32;  // 16-byte loads per lane per row
     constexpr int RW = BS / 4;   // rows per wave
     constexpr int NG = RW / 4;   // 4-row groups per wave
-    constexpr int LPV = 16 / NG; // lanes per row total after the reduction
     __shared__ double colred[16][BS];
 
     const i64 id = blockIdx.x;
@@ -103,29 +102,47 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
         cacc[2 * s] = 0.0;
         cacc[2 * s + 1] = 0.0;
     }
-    double p[NG];
+    // GB row groups per batch: a batch's loads are issued together, then its
+    // GB row sums are finished (16-lane reduction) and written, so only one
+    // batch of loads and sums is live at a time (ROLL keeps the batch loop
+    // rolled, which stops the compiler from hoisting later batches' loads).
+    // Measured: b = 128 streams fastest with all 8 groups in one batch (2
+    // waves per SIMD, 128 KB in flight per work-group); b = 256 with batches
+    // of 2 (3 waves per SIMD; a whole block in registers spills).
+#pragma unroll ROLL
+    for (int g0 = 0; g0 < NG; g0 += GB) {
+        double a[GB][2 * NS];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const int row = wave * RW + 4 * g + q;
-        const double* rp = blk + (i64)row * BS + 2 * c16;
-        const double xi = xI[row];
-        double acc = 0.0;
+        for (int g = 0; g < GB; ++g) {
+            const double* rp = blk + (i64)(wave * RW + 4 * (g0 + g) + q) * BS + 2 * c16;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            double a0 = __builtin_nontemporal_load(rp + 32 * s);
-            double a1 = __builtin_nontemporal_load(rp + 32 * s + 1);
-            if (ABS) {
-                a0 = fabs(a0);
-                a1 = fabs(a1);
+            for (int s = 0; s < NS; ++s) {
+                a[g][2 * s] = __builtin_nontemporal_load(rp + 32 * s);
+                a[g][2 * s + 1] = __builtin_nontemporal_load(rp + 32 * s + 1);
             }
-            acc += a0 * xj[2 * s] + a1 * xj[2 * s + 1];
-            cacc[2 * s] += a0 * xi;
-            cacc[2 * s + 1] += a1 * xi;
         }
-        p[g] = acc;
+        double pb[GB];
+#pragma unroll
+        for (int g = 0; g < GB; ++g) {
+            const double xi = xI[wave * RW + 4 * (g0 + g) + q];
+            double acc = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double a0 = a[g][2 * s], a1 = a[g][2 * s + 1];
+                if (ABS) {
+                    a0 = fabs(a0);
+                    a1 = fabs(a1);
+                }
+                acc += a0 * xj[2 * s] + a1 * xj[2 * s + 1];
+                cacc[2 * s] += a0 * xi;
+                cacc[2 * s + 1] += a1 * xi;
+            }
+            pb[g] = acc;
+        }
+        constexpr int LPB = 16 / GB;  // lanes per row total after the reduction
+        const double tot = halving_reduce<GB, 16>(pb);
+        if ((c16 & (LPB - 1)) == 0) rowpart[id * BS + wave * RW + 4 * (g0 + c16 / LPB) + q] = tot;
     }
-    const double tot = halving_reduce<NG, 16>(p);
-    if ((c16 & (LPV - 1)) == 0) rowpart[id * BS + wave * RW + 4 * (c16 / LPV) + q] = tot;
     if (diag) return;  // diagonal blocks hold both triangles: row form only
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -317,9 +334,9 @@ static int launch_partials_abs(hs_handle_t h, const double* A, const double* x, 
     const i64 blocks = (i64)n * (n + 1) / 2;
     dim3 grid((unsigned)blocks), blk(SYMV_THREADS);
     switch (b) {
-        case 64: symv_partials_kernel<64, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
-        case 128: symv_partials_kernel<128, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
-        case 256: symv_partials_kernel<256, true><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        case 64: symv_partials_kernel<64, true, 4><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        case 128: symv_partials_kernel<128, true, 8><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
+        case 256: symv_partials_kernel<256, true, 2><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, nullptr, 0); break;
         default: hs_set_error("SYMV: unsupported block size"); return HS_ERR_UNSUPPORTED;
     }
     HS_LAUNCH_CHECK();
@@ -335,9 +352,9 @@ static int launch_partials(hs_handle_t h, const double* A, const double* x, doub
     }
     dim3 grid((unsigned)blocks), blk(SYMV_THREADS);
     switch (b) {
-        case 64: symv_partials_kernel<64><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
-        case 128: symv_partials_kernel<128><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
-        case 256: symv_partials_kernel<256><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        case 64: symv_partials_kernel<64, false, 4><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        case 128: symv_partials_kernel<128, false, 8><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
+        case 256: symv_partials_kernel<256, false, 2><<<grid, blk, 0, h->stream>>>(A, x, rowpart, colpart, n, stop, it); break;
         default: hs_set_error("SYMV: unsupported block size"); return HS_ERR_UNSUPPORTED;
     }
     HS_LAUNCH_CHECK();
