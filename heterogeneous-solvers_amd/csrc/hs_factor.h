@@ -159,19 +159,13 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, in
     }
 }
 
-// Work-group order of a trailing-update launch; HS_TRAIL_ORDER (bit 0 band
-// order, bit 1 XCD chunking, default both) and HS_TRAIL_BAND (band height,
-// default 4) select it for experiments.
+// Work-group order of a trailing-update launch: band order (bit 0) and XCD
+// chunking (bit 1), bands of 4 block rows. The FP64 update ran equally fast
+// with and without them (it is bound by MFMA issue, not by its 1.9-2.5 TB/s
+// of memory traffic); both stay on.
 inline void hs_trail_order(int* bits, int* band) {
-    static int b0 = -1, h0 = 4;
-    if (b0 < 0) {
-        const char* e = std::getenv("HS_TRAIL_ORDER");
-        b0 = e ? std::atoi(e) : 3;
-        const char* hb = std::getenv("HS_TRAIL_BAND");
-        h0 = hb ? std::max(1, std::atoi(hb)) : 4;
-    }
-    *bits = b0;
-    *band = h0;
+    *bits = 3;
+    *band = 4;
 }
 
 // Fills the pair map of a trailing-update launch with tilesPerBlock work-
