@@ -69,15 +69,18 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 #pragma unroll
         for (int j = 0; j < MT; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    // np panels: panel p + 1 of a block row starts n - k - 1 blocks after
-    // panel p (column-major block order), the same step for every row
+    // np panels: in column-major block order, block (I, k+p) lies
+    // p (n-k-1) - p (p-1)/2 blocks after block (I, k), the same for every row
     const int nk = b / KT, nkt = nk * pm.np;
-    const i64 pstep = (i64)(n - pm.k - 1) * bb;
+    auto poff = [&](int kt) -> i64 {
+        const i64 pp = kt / nk;
+        return (pp * (n - pm.k - 1) - pp * (pp - 1) / 2) * bb + (i64)(kt % nk) * KT;
+    };
     auto gload = [&](int kt) {
 #pragma unroll
         for (int v = 0; v < LPT / 2; ++v) {
-            ra[v] = *reinterpret_cast<const double2*>(pa + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 2 * v);
-            rb[v] = *reinterpret_cast<const double2*>(pb + (kt >= nk ? pstep : 0) + (kt % nk) * KT + 2 * v);
+            ra[v] = *reinterpret_cast<const double2*>(pa + poff(kt) + 2 * v);
+            rb[v] = *reinterpret_cast<const double2*>(pb + poff(kt) + 2 * v);
         }
     };
     auto sstore = [&](int buf) {

@@ -19,11 +19,12 @@
 //  mode 3: rows k+1.., J in [k+1, I],   offset pairOffset    (whole step)
 //  mode 4: rows and columns from rs = k + 2 np, J <= I       (look-ahead rest)
 //  mode 5: J = rs = k+1, I = rs + q                          (look-ahead panel column)
-//  mode 6: J in {rs, rs+1} with rs = k + np, I >= J          (two look-ahead columns)
+//  mode 6: J in [rs, rs+nc) with rs = rowStart, nc = rowCount, I >= J
+//                                                            (look-ahead columns)
 struct PairMap {
     int k, rowStart, mode;
     i64 pairOffset;
-    int n = 0, np = 1;
+    int n = 0, np = 1, nc = 1;
     // band order (modes 3 and 4 over a whole triangle): m rows, bands of
     // `band` block rows walked column by column; m == 0 keeps pair_of's order
     int m = 0, band = 4;
@@ -45,14 +46,15 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
         return;
     }
     if (pm.mode == 6) {
-        const int m1 = pm.n - pm.rowStart;
-        if (q < m1) {
-            *I = pm.rowStart + (int)q;
-            *J = pm.rowStart;
-        } else {
-            *I = pm.rowStart + 1 + (int)(q - m1);
-            *J = pm.rowStart + 1;
+        // columns rs .. rs+nc-1, column c holding rows rs+c .. n-1
+        int c = 0;
+        i64 r = q;
+        while (c + 1 < pm.nc && r >= pm.n - pm.rowStart - c) {
+            r -= pm.n - pm.rowStart - c;
+            ++c;
         }
+        *J = pm.rowStart + c;
+        *I = *J + (int)r;
         return;
     }
     int r, c;
@@ -140,8 +142,8 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, in
         *rs = k + 1;
         *pairs = std::max(0, n - *rs);
     } else if (mode == 6) {
-        *rs = k + np;
-        *pairs = std::max(0, n - *rs) + std::max(0, n - *rs - 1);
+        *pairs = 0;
+        for (int c = 0; c < rowCount; ++c) *pairs += std::max(0, n - rowStart - c);
     } else if (mode == 2) {
         const int firstRow = std::max(rowStart, k + 2);
         const int lastRow = rowStart + rowCount;
@@ -177,6 +179,7 @@ inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int n
     *pm = PairMap{k, rs, mode, off};
     pm->n = n;
     pm->np = np;
+    pm->nc = (mode == 6) ? std::max(1, std::min(rowCount, n - rs)) : 1;
     if (pairs <= 0) return 0;
     int bits, band;
     hs_trail_order(&bits, &band);
@@ -244,15 +247,17 @@ struct PhaseTimer {
     }
 };
 
-// Panels per step of the look-ahead schedule: 2 (trailing updates of depth
-// 2b) unless HS_PANELS=1, and 1 for fewer than 4 block columns.
+// Panels per step of the look-ahead schedule: 4 (trailing updates of depth
+// 4b; measured at N = 65536, b = 256: 1569 / 1526 / 1506 ms for 2 / 3 / 4),
+// HS_PANELS = 1..4 to choose another, and never more than n/2.
 inline int hs_schedule_panels(int n) {
     static int v = -1;
     if (v < 0) {
         const char* e = std::getenv("HS_PANELS");
-        v = (e && std::atoi(e) == 1) ? 1 : 2;
+        const int w = e ? std::atoi(e) : 0;
+        v = (w >= 1 && w <= 4) ? w : 4;
     }
-    return n >= 4 ? v : 1;
+    return std::max(1, std::min(v, n / 2));
 }
 
 // Launches and algorithmic flops of the rest-update launches (stream S1) of
@@ -276,10 +281,10 @@ inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops) 
 
 // Right-looking factorisation with a look-ahead of one step on two streams of
 // the same priority; a step applies P panels (P = hs_schedule_panels(n)).
-// Stream S2 factors panels: the step's columns k+P .. k+2P-1 get the update
-// by panels k .. k+P-1 (trail mode 5 for P = 1, mode 6 for P = 2), then for
-// each of the next P panels POTRF and TRSM, the second panel's column first
-// updated by the first one (mode 5). Stream S1 applies the step's panels to
+// Stream S2 factors panels: the next step's columns k+P .. k+2P-1 get the
+// update by panels k .. k+P-1 (trail mode 5 for P = 1, mode 6 otherwise),
+// then for each of those P panels POTRF and TRSM, each panel updating the
+// step's later columns before their POTRF (modes 5 and 6 with one panel). Stream S1 applies the step's panels to
 // the rest of the trailing triangle (mode 4, rows and columns >= k + 2P) as
 // soon as they are final, so the next step's panel work runs underneath it.
 // S1's update of step s waits for the panels of step s (event P[s]); S2's
@@ -298,15 +303,17 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
     // S2 must see everything already queued on S1 (e.g. the input assembly)
     HS_HIP(hipEventRecord(evU[n], s1));
     HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
-    // POTRF and TRSM of panels k0 .. k0+P-1 on S2
+    // POTRF and TRSM of panels k0 .. k0+P-1 on S2, right-looking inside the
+    // step: panel k updates the step's later columns k+1 .. k0+P-1 at once
     auto chain = [&](int k0) -> int {
         OnStream os(h, s2);
         int r;
-        for (int k = k0; k < std::min(n, k0 + P); ++k) {
-            if (k > k0 && (r = trail(s2, k - 1, 5, k, n - k, 1))) return r;
+        const int kend = std::min(n, k0 + P);
+        for (int k = k0; k < kend; ++k) {
             if ((r = pt.begin(s2, 0)) || (r = potrf(k)) || (r = pt.end(s2, 0))) return r;
             if (k + 1 < n && ((r = pt.begin(s2, 1)) || (r = trsm(k, k + 1, n - k - 1)) || (r = pt.end(s2, 1))))
                 return r;
+            if (k + 1 < kend && (r = trail(s2, k, k + 2 == kend ? 5 : 6, k + 1, kend - k - 1, 1))) return r;
         }
         return HS_OK;
     };
@@ -315,7 +322,8 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
     int step = 0;
     for (int k = 0; k + P < n; k += P, ++step) {
         if (step >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[step - 1], 0));
-        if ((rc = trail(s2, k, P == 1 ? 5 : 6, k + P, n - k - P, P))) return rc;
+        // columns k+P .. k+2P-1 by the step's P panels
+        if ((rc = trail(s2, k, P == 1 ? 5 : 6, k + P, P == 1 ? n - k - 1 : P, P))) return rc;
         if ((rc = chain(k + P))) return rc;
         HS_HIP(hipEventRecord(evP[step + 1], s2));
         HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
