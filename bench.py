@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # average over the look-ahead rest-update launches at N = 65536, b = 256):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors that make the
 # counters read 1 GiB for a copy kernel of 1 GiB read + 1 GiB written
-# (bench.py --copy-calibration); two-panel steps, 126 launches. Summary and
+# (bench.py --copy-calibration); four-panel steps, 62 launches. Summary and
 # commands: profiles/trailing_mfma_f64_cholesky65536_hbm_traffic.md
-TRAILING_TRAFFIC_BYTES = 23.41e9
+TRAILING_TRAFFIC_BYTES = 41.00e9
 
 
 def _load(name, path):
