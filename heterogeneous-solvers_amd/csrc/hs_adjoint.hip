@@ -199,6 +199,136 @@ __global__ __launch_bounds__(ATHREADS) void adj_symm_kernel(const double* __rest
     for_each_out(acc, wave, lane, [&](int r, int cc, double v) { Wt[(i64)(r0 + r) * b + c0 + cc] = v; });
 }
 
+// The same product in 128 x 128 tiles for b % 128 == 0: the trailing-update
+// kernel's structure (2 x 2 waves of 64 x 64 f64 MFMA accumulators, 16-deep
+// k stages double-buffered in LDS with the next stage's global loads in
+// flight), with the three A layouts of Sfull (S_IJ row-major, S_JI^T read
+// column-wise, S_II + S_II^T) and the panel block P_J read row-wise as B.
+constexpr int ST = 128;               // tile edge
+constexpr int SKT = 16;               // k stage depth
+constexpr int SLDP = ST + 4;          // LDS pitch (doubles)
+constexpr int SSTAGE = SKT * SLDP;
+
+__global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restrict__ Ab, const double* __restrict__ L,
+                                                          double* __restrict__ W, int b, int n, int k, int m,
+                                                          int chunk, int tilesN) {
+    __shared__ double As[2 * SSTAGE];
+    __shared__ double Bs[2 * SSTAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tilesPer = tilesN * tilesN;
+    const int tile = blockIdx.x % tilesPer;
+    const int rest = blockIdx.x / tilesPer;
+    const int ip = rest % m;
+    const int c = rest / m;
+    const int I = k + 1 + ip;
+    const int Ja = k + 1 + c * chunk, Jb = std::min(n, Ja + chunk);
+    const int tm = tile / tilesN, tn = tile % tilesN;
+    const i64 bb = (i64)b * b;
+    const int nkb = b / SKT;                  // stages per J block
+    const int nst = (Jb - Ja) * nkb;          // stages of this work-group
+    // thread -> staging positions: row-major A (8 consecutive k of one row)
+    // and column-wise A / B (8 consecutive m of one k row)
+    const int lrow = tid >> 1, lk = (tid & 1) * 8;
+    const int kr = tid >> 4, m0 = (tid & 15) * 8;
+    double2 ra[4], ra2[4], rb[4];
+    int sJ = 0;  // J of the stage held in registers
+    auto gload = [&](int st) {
+        const int J = Ja + st / nkb, t0 = (st % nkb) * SKT;
+        sJ = J;
+        if (J <= I) {
+            const double* p = Ab + hs_block_id(I, J, n) * bb + (i64)(tm * ST + lrow) * b + t0 + lk;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ra[v] = *reinterpret_cast<const double2*>(p + 2 * v);
+        }
+        if (J >= I) {
+            const double* p = Ab + hs_block_id(J, I, n) * bb + (i64)(t0 + kr) * b + tm * ST + m0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ra2[v] = *reinterpret_cast<const double2*>(p + 2 * v);
+        }
+        const double* pb = L + hs_block_id(J, k, n) * bb + (i64)(t0 + kr) * b + tn * ST + m0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rb[v] = *reinterpret_cast<const double2*>(pb + 2 * v);
+    };
+    auto sstore = [&](int buf) {
+        double* as = As + buf * SSTAGE;
+        double* bs = Bs + buf * SSTAGE;
+        if (sJ < I) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                as[(lk + 2 * v) * SLDP + lrow] = ra[v].x;
+                as[(lk + 2 * v + 1) * SLDP + lrow] = ra[v].y;
+            }
+        } else if (sJ > I) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                as[kr * SLDP + m0 + 2 * v] = ra2[v].x;
+                as[kr * SLDP + m0 + 2 * v + 1] = ra2[v].y;
+            }
+        } else {
+            // S_II + S_II^T: the row-major part first, then the transposed
+            // part added after a barrier (uniform: sJ is the same in the group)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                as[(lk + 2 * v) * SLDP + lrow] = ra[v].x;
+                as[(lk + 2 * v + 1) * SLDP + lrow] = ra[v].y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                as[kr * SLDP + m0 + 2 * v] += ra2[v].x;
+                as[kr * SLDP + m0 + 2 * v + 1] += ra2[v].y;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            bs[kr * SLDP + m0 + 2 * v] = rb[v].x;
+            bs[kr * SLDP + m0 + 2 * v + 1] = rb[v].y;
+        }
+    };
+    ad4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = ad4{0.0, 0.0, 0.0, 0.0};
+    const int lr = lane & 15, lq = lane >> 4;
+    const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 64;
+    gload(0);
+    sstore(0);
+    if (nst > 1) gload(1);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const double* as = As + (st & 1) * SSTAGE;
+        const double* bs = Bs + (st & 1) * SSTAGE;
+#pragma unroll
+        for (int ks = 0; ks < SKT; ks += 4) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = as[(ks + lq) * SLDP + wm0 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = bs[(ks + lq) * SLDP + wn0 + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (st + 1 < nst) {
+            sstore((st + 1) & 1);
+            if (st + 2 < nst) gload(st + 2);
+        }
+        __syncthreads();
+    }
+    // output: register v of lane l is row l/16 + 4v, column l%16 of its tile
+    double* Wt = W + ((i64)c * m + ip) * bb + (i64)(tm * ST) * b + tn * ST;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                Wt[(i64)(wm0 + i * 16 + lq + 4 * v) * b + wn0 + j * 16 + lr] = acc[i][j][v];
+}
+
 // dst[e] -= sum_{c < nch} W[c * stride + e], e < count (fixed order)
 __global__ void adj_sub_partials_kernel(double* __restrict__ dst, const double* __restrict__ W, i64 count, i64 stride,
                                         int nch) {
@@ -364,7 +494,6 @@ inline int symm_chunk(int m, int tiles) {
     const int nch = std::max(1, std::min(m, (2048 + m * tiles - 1) / (m * tiles)));
     return (m + nch - 1) / nch;
 }
-constexpr int GRAM_CHUNK = 8;  // panel blocks per partial of Abar_panel^T P
 
 // Workspace layout (doubles) of one gradient evaluation.
 struct AdjWork {
@@ -373,7 +502,7 @@ struct AdjWork {
 
 AdjWork adj_layout(int n, int b) {
     const i64 bb = (i64)b * b, nb = (i64)n * b;
-    const int tiles = (b / AT) * (b / AT);
+    const int tiles = (b % ST == 0) ? (b / ST) * (b / ST) : (b / AT) * (b / AT);
     i64 wp = 0;
     for (int k = 0; k + 1 < n; ++k) {
         const int m = n - k - 1;
@@ -397,7 +526,7 @@ AdjWork adj_layout(int n, int b) {
     w.grad = take(4);
     w.wp = take(std::max<i64>(wp, 1));
     w.wq = take((i64)std::max(1, n - 1) * bb);
-    w.wg = take((i64)((n + GRAM_CHUNK - 1) / GRAM_CHUNK) * bb);
+    w.wg = take((i64)std::max(1, n - 1) * bb);
     w.t1 = take(bb);
     w.t2 = take(bb);
     w.t3 = take(bb);
@@ -462,10 +591,16 @@ int reverse(hs_handle_t h, const double* L, double* Ab, i64 N, int b, double* ws
             double* Pbar = Ab + hs_block_id(k + 1, k, n) * bb;     // panel adjoint, m blocks
             const double* P = L + hs_block_id(k + 1, k, n) * bb;   // panel, m blocks
             // Pbar -= (S + S^T) P
-            const int ch = symm_chunk(m, tiles);
+            const bool big = (b % ST == 0);
+            const int stn = big ? b / ST : tilesN, stiles = stn * stn;
+            const int ch = symm_chunk(m, stiles);
             const int nch = (m + ch - 1) / ch;
-            adj_symm_kernel<<<dim3((unsigned)((i64)nch * m * tiles)), dim3(ATHREADS), 0, h->stream>>>(
-                Ab, L, ws + w.wp, b, n, k, m, ch, tilesN);
+            if (big)
+                adj_symm128_kernel<<<dim3((unsigned)((i64)nch * m * stiles)), dim3(256), 0, h->stream>>>(
+                    Ab, L, ws + w.wp, b, n, k, m, ch, stn);
+            else
+                adj_symm_kernel<<<dim3((unsigned)((i64)nch * m * stiles)), dim3(ATHREADS), 0, h->stream>>>(
+                    Ab, L, ws + w.wp, b, n, k, m, ch, stn);
             HS_LAUNCH_CHECK();
             adj_sub_partials_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wp, m * bb,
                                                                                       m * bb, nch);
@@ -475,15 +610,11 @@ int reverse(hs_handle_t h, const double* L, double* Ab, i64 N, int b, double* ws
             if ((rc = launch_gemm(h, g1, m, tilesN))) return rc;
             adj_copy_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wq, m * bb);
             HS_LAUNCH_CHECK();
-            // Dbar -= tril(Abar_panel^T P), partials over GRAM_CHUNK panel blocks
-            const int ng = (m + GRAM_CHUNK - 1) / GRAM_CHUNK;
-            for (int q = 0; q < ng; ++q) {
-                const int cnt = std::min(GRAM_CHUNK, m - q * GRAM_CHUNK);
-                GemmArgs g2{Pbar + (i64)q * GRAM_CHUNK * bb, 0, b, 1, P + (i64)q * GRAM_CHUNK * bb, 0, b,
-                            ws + w.wg + (i64)q * bb, 0, b, cnt * b, tilesN, tiles, 0};
-                if ((rc = launch_gemm(h, g2, 1, tilesN))) return rc;
-            }
-            adj_gram_sub_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, ng);
+            // Dbar -= tril(Abar_panel^T P): one partial per panel block in a
+            // single batched launch, summed in a fixed order
+            GemmArgs g2{Pbar, bb, b, 1, P, bb, b, ws + w.wg, bb, b, b, tilesN, tiles, 0};
+            if ((rc = launch_gemm(h, g2, m, tilesN))) return rc;
+            adj_gram_sub_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, m);
             HS_LAUNCH_CHECK();
         }
         // POTRF reverse on block k
