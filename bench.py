@@ -194,7 +194,7 @@ def mixed_workload(hs, dev, syn, args):
     return out
 
 
-def cg_workload(hs, dev, syn, args):
+def cg_workload(hs, dev, syn, args, symv_only=False):
     seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cg"]
     n = -(-N // b)
     X, y = syn.gp_inputs(seed, N, d)
@@ -216,6 +216,12 @@ def cg_workload(hs, dev, syn, args):
     symv_ms = timer.ms() / reps
     passes = (C.c_double * 2)()
     hs.call("hs_symv_packed_f64_timed", dev.h, dA.p, drhs.p, dq.p, N, b, reps, passes)
+    bytes_symv = plen * 8 + 16 * n * b
+    if symv_only:
+        dA.free()
+        return {"N": N, "b": b, "symv_ms": symv_ms, "symv_gbs": bytes_symv / (symv_ms / 1e3) / 1e9,
+                "symv_block_pass_ms": passes[0], "symv_reduce_pass_ms": passes[1],
+                "symv_block_pass_gbs": bytes_symv / (passes[0] / 1e3) / 1e9}
     it, df, d0 = C.c_int64(), C.c_double(), C.c_double()
     timer.start()
     hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100000, C.byref(it), C.byref(df), C.byref(d0))
@@ -227,7 +233,6 @@ def cg_workload(hs, dev, syn, args):
     hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dq.p, N, b)
     rel_res = float(np.linalg.norm(rhs - dev.download(dq, n * b)) / np.linalg.norm(rhs))
     dA.free()
-    bytes_symv = plen * 8 + 16 * n * b
     log("cg workload N=%d b=%d: %d iterations in %.1f ms, symv %.3f ms (%.0f GB/s); block pass %.3f ms "
         "(%.0f GB/s), reduction pass %.3f ms" % (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6,
                                                  passes[0], bytes_symv / passes[0] / 1e6, passes[1]))
@@ -331,6 +336,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--only", choices=["cg", "symv", "adjoint"],
+                    help="run only this side workload and print its record (for profiling; not a bench line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a kernel of known HBM traffic (y = 2x over 2^27 doubles, 5 launches) to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -352,6 +359,14 @@ def main():
 
     hs = _load("hs_amd", os.path.join(PKG, "hs_amd.py"))
     syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
+    if args.only:
+        with hs.Device(0) as dev:
+            if args.only == "adjoint":
+                rec = adjoint_workload(hs, dev, syn, args)
+            else:
+                rec = cg_workload(hs, dev, syn, args, symv_only=(args.only == "symv"))
+        print(json.dumps({args.only: rec}))
+        return
     with hs.Device(0) as dev:
         ch = cholesky_workload(hs, dev, syn, args)
         mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args)

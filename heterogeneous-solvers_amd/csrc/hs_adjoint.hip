@@ -204,6 +204,8 @@ __global__ __launch_bounds__(ATHREADS) void adj_symm_kernel(const double* __rest
 // k stages double-buffered in LDS with the next stage's global loads in
 // flight), with the three A layouts of Sfull (S_IJ row-major, S_JI^T read
 // column-wise, S_II + S_II^T) and the panel block P_J read row-wise as B.
+// The partial tiles are written once and read once by the summing pass:
+// non-temporal stores keep them from displacing operands in the cache.
 constexpr int ST = 128;               // tile edge
 constexpr int SKT = 16;               // k stage depth
 constexpr int SLDP = ST + 4;          // LDS pitch (doubles)
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restri
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                Wt[(i64)(wm0 + i * 16 + lq + 4 * v) * b + wn0 + j * 16 + lr] = acc[i][j][v];
+                __builtin_nontemporal_store(acc[i][j][v], Wt + (i64)(wm0 + i * 16 + lq + 4 * v) * b + wn0 + j * 16 + lr);
 }
 
 // dst[e] -= sum_{c < nch} W[c * stride + e], e < count (fixed order)
