@@ -308,15 +308,17 @@ __global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restri
             for (int i = 0; i < 4; ++i) av[i] = as[(ks + lq) * SLDP + wm0 + i * 16 + lr];
 #pragma unroll
             for (int j = 0; j < 4; ++j) bv[j] = bs[(ks + lq) * SLDP + wn0 + j * 16 + lr];
+            // the other buffer was last read in stage st-1, which every wave
+            // left at the barrier: fill it between this stage's MFMAs
+            if (ks == 0 && st + 1 < nst) {
+                sstore((st + 1) & 1);
+                if (st + 2 < nst) gload(st + 2);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
-        }
-        if (st + 1 < nst) {
-            sstore((st + 1) & 1);
-            if (st + 2 < nst) gload(st + 2);
         }
         __syncthreads();
     }

@@ -159,15 +159,16 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
             for (int i = 0; i < MT; ++i) av[i] = as[(ks + lq) * LDP + wm0 + i * 16 + lr];
 #pragma unroll
             for (int j = 0; j < MT; ++j) bv[j] = bs[(ks + lq) * LDP + wn0 + j * 16 + lr];
+            // the other buffer was last read in stage kt-1 (see trailing_mfma_f64)
+            if (ks == 0 && kt + 1 < nkt) {
+                sstore((kt + 1) & 1);
+                if (kt + 2 < nkt) gload(kt + 2);
+            }
 #pragma unroll
             for (int i = 0; i < MT; ++i)
 #pragma unroll
                 for (int j = 0; j < MT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
-        }
-        if (kt + 1 < nkt) {
-            sstore((kt + 1) & 1);
-            if (kt + 2 < nkt) gload(kt + 2);
         }
         __syncthreads();
     }
@@ -285,6 +286,12 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
             bh[j] = *reinterpret_cast<const bf8*>(s + 2 * PLANE + off);
             bl[j] = *reinterpret_cast<const bf8*>(s + 3 * PLANE + off);
         }
+        // the other buffer was last read in stage kt-1: fill it while this
+        // stage's MFMAs run
+        if (kt + 1 < nkt) {
+            sstore((kt + 1) & 1);
+            if (kt + 2 < nkt) gload(kt + 2);
+        }
 #pragma unroll
         for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -293,10 +300,6 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
             }
-        if (kt + 1 < nkt) {
-            sstore((kt + 1) & 1);
-            if (kt + 2 < nkt) gload(kt + 2);
-        }
         __syncthreads();
     }
     epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
