@@ -1,8 +1,10 @@
 // Command-line front end with the reference's flags (src/main.cpp:37-187) and
 // the same flow (main.cpp:189-252): parse or generate A and b, build the load
-// balancer, run CG or Cholesky + triangular solves, write metrics.json.
+// balancer, run CG or Cholesky + triangular solves (or, with --gpr, the
+// Gaussian-process prediction after the solve), write metrics.json.
 // Flags take the form --name=value or --name value.
 #include <cmath>
+#include <iomanip>
 #include <iostream>
 #include <map>
 #include <memory>
@@ -47,6 +49,9 @@ int main(int argc, char** argv) {
         if (a.count("algorithm")) conf::algorithm = a["algorithm"];
         if (a.count("print_verbose")) conf::printVerbose = toBool(a["print_verbose"]);
         if (a.count("check_result")) conf::checkResult = toBool(a["check_result"]);
+        if (a.count("test_size")) conf::N_test = std::stoul(a["test_size"]);
+        std::string gpTest = a.count("gp_test") ? a["gp_test"] : "";
+        const bool performGPR = a.count("gpr") && toBool(a["gpr"]);
         const bool generate = !gpIn.empty();
 
         hs::Queue gpuQueue(hs::gpu_selector_v);
@@ -65,6 +70,17 @@ int main(int argc, char** argv) {
                                                        A.blockCountXY);
         std::vector<double> b0(b.rightHandSideData.begin(), b.rightHandSideData.end());
         std::vector<double> xres;
+        if (performGPR) {
+            // main.cpp:222-224: solve, then predict at the test inputs
+            if (!generate) throw std::invalid_argument("--gpr needs --gp_input and --gp_output");
+            GaussianProcess GP(A, b, gpIn, gpTest, cpuQueue, gpuQueue, lb);
+            GP.start();
+            double s = 0.0;
+            for (double v : GP.result) s += v;
+            std::cout << "RESULT algorithm=gpr test_size=" << GP.result.size() << " mean_sum=" << std::setprecision(17)
+                      << s << std::endl;
+            return 0;
+        }
         if (conf::algorithm == "cg") {
             CG cg(A, b, cpuQueue, gpuQueue, lb);
             cg.solveHeterogeneous();

@@ -35,11 +35,14 @@ void requireFullGPUShare(const LoadBalancer& lb, const char* who) {
     }
 }
 
+// UtilityFunctions::writeResult (UtilityFunctions.cpp:19-33): one line, values
+// separated by spaces, fixed notation with 20 digits
 template <typename V>
 void writeResult(const std::string& dir, const V& x) {
     std::ofstream out(dir + "/x_result.txt");
-    out << std::setprecision(20) << std::scientific;
-    for (std::size_t i = 0; i < x.size(); ++i) out << x[i] << "\n";
+    out << std::setprecision(20) << std::fixed;
+    for (std::size_t i = 0; i < x.size(); ++i) out << x[i] << " ";
+    out << std::endl;
 }
 
 }  // namespace

@@ -1,15 +1,20 @@
 // Probe program for the C++ host API, driven by tests/test_host_api.py.
 // Subcommands print what the pytest side compares with the reference's
-// golden values (no GPU needed for any of them):
+// golden values (no GPU needed for the first six):
 //   layout <matrix.txt> <bsz> <idx>...   values of A.matrixData at idx
 //   shape <matrix.txt> <bsz>             blockSize blockCountXY size
 //   rowparse <string>                    parsed values or invalid_argument
 //   rhs <vector.txt> <bsz>               padded right-hand side
 //   lb <share>                           StaticLoadBalancer validation
 //   cgdims                               CG dimension checks
+//   gpcpu                                matrixVectorGP on a CPU queue (rejected)
+// and, on a GPU:
+//   gemvgp <m> <n>                       matrixVectorGP on exact binary data
+//   kstar <train> <test> <N> <M>         generateTestKernelMatrix, all entries
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "hs/hs.hpp"
 
@@ -69,6 +74,39 @@ int main(int argc, char** argv) {
                     std::cout << "invalid_argument " << e.what() << std::endl;
                 }
             }
+            return 0;
+        }
+        if (cmd == "gpcpu") {
+            double A[4] = {1, 2, 3, 4}, x[2] = {1, 1}, y[2];
+            try {
+                MatrixVectorOperations::matrixVectorGP(host, A, x, y, 2, 2);
+                std::cout << "no error" << std::endl;
+            } catch (const std::invalid_argument& e) {
+                std::cout << "invalid_argument " << e.what() << std::endl;
+            }
+            return 0;
+        }
+        if (cmd == "gemvgp") {
+            // entries are small multiples of 1/8 and 1/4: every product and
+            // partial sum is exact, so any summation order gives the same bits
+            const int m = std::stoi(argv[2]), n = std::stoi(argv[3]);
+            std::vector<double> A((std::size_t)m * n), x(n), y(m);
+            for (int r = 0; r < m; ++r)
+                for (int k = 0; k < n; ++k) A[(std::size_t)r * n + k] = ((r * 7 + k * 3) % 11 - 5) * 0.125;
+            for (int k = 0; k < n; ++k) x[k] = (k % 13 - 6) * 0.25;
+            hs::Queue gpu(hs::gpu_selector_v);
+            MatrixVectorOperations::matrixVectorGP(gpu, A.data(), x.data(), y.data(), n, m);
+            for (double v : y) std::cout << v << "\n";
+            return 0;
+        }
+        if (cmd == "kstar") {
+            std::string train = argv[2], test = argv[3];
+            conf::N = std::stoul(argv[4]);
+            conf::N_test = std::stoul(argv[5]);
+            std::vector<double> K(conf::N * conf::N_test);
+            hs::Queue gpu(hs::gpu_selector_v);
+            MatrixGenerator::generateTestKernelMatrix(train, test, host, gpu, K.data());
+            for (double v : K) std::cout << v << "\n";
             return 0;
         }
     } catch (const std::exception& e) {

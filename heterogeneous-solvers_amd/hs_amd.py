@@ -73,6 +73,10 @@ _SIGS = {
     "hs_trsv_packed_f64": [_P, _P, _P, _L, _I, _I],
     "hs_gp_nll_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P, _P, C.POINTER(_D)],
     "hs_gp_nll_grad_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P, _P, _P, C.POINTER(_D), C.POINTER(_D)],
+    "hs_rbf_cross_f64": [_P, _P, _L, _P, _L, _I, _D, _D, _P],
+    "hs_gemv_rows_f64": [_P, _P, _P, _P, _L, _L],
+    "hs_gp_predict_f64": [_P, _P, _L, _P, _L, _I, _D, _D, _P, _P],
+    "hs_window_features_f64": [_P, _P, _L, _P],
 }
 
 HS_MIXED_FP32 = 0

@@ -18,4 +18,9 @@ public:
                                              double signalVariance, double noiseVariance, int blockSize,
                                              hs::Queue& queueGPU);
     static RightHandSide parseRHS_GP(std::string& path, hs::Queue& queue);
+    // K*[j * conf::N + i] for the conf::N training and conf::N_test test inputs
+    // (8-lag windows of the two series, sigma_f^2 = 1, l = 1) into the host
+    // array K_star, computed on queueGPU (MatrixGenerator.cpp:163-213).
+    static void generateTestKernelMatrix(std::string& path_train, std::string& path_test, hs::Queue& queue,
+                                         hs::Queue& queueGPU, conf::fp_type* K_star);
 };

@@ -250,6 +250,39 @@ int hs_gp_nll_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, doubl
 int hs_gp_nll_grad_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, double ell, double sf2, double sn2,
                        const double* y, double* K, double* Kbar, double* nll, double* grad);
 
+/* ---- Gaussian-process prediction (GaussianProcess::start after the solve,
+ * GaussianProcess.cpp:42-59) ---- */
+
+/* Test/training kernel matrix on the device, the reference's transposed
+ * layout: Kstar[j * N + i] = sf2 * exp(-0.5 / ell^2 * sum_k (Xtrain[i,k] -
+ * Xtest[j,k])^2), distance summed in feature order, for i < N training rows
+ * and j < M test rows (both row-major with d <= 64 features, on the device).
+ * Replaces the CPU-queue kernel of MatrixGenerator::generateTestKernelMatrix
+ * (MatrixGenerator.cpp:163-213), which fixes sf2 = ell = 1. */
+int hs_rbf_cross_f64(hs_handle_t h, const double* Xtrain, int64_t N, const double* Xtest, int64_t M, int d,
+                     double ell, double sf2, double* Kstar);
+
+/* y_r = sum_k A[r * n + k] x_k for r < m (A row-major m x n, device).
+ * Replaces MatrixVectorOperations::matrixVectorGP (MatrixVectorOperations.cpp:
+ * 379-395); the sum is a fixed-order tree, not the reference's sequential
+ * loop. */
+int hs_gemv_rows_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t n, int64_t m);
+
+/* Predictive mean mean_j = sum_i sf2 exp(-0.5/ell^2 ||Xtest_j - Xtrain_i||^2)
+ * alpha_i for j < M: hs_rbf_cross_f64 followed by hs_gemv_rows_f64 in one
+ * pass that never stores K*. Deterministic (fixed split of the training rows
+ * and fixed summation order). alpha (device) holds N values, mean (device) M.
+ * Replaces the K* assembly + matrixVectorGP pair of GaussianProcess::start
+ * (GaussianProcess.cpp:47-56). */
+int hs_gp_predict_f64(hs_handle_t h, const double* Xtrain, int64_t N, const double* Xtest, int64_t M, int d,
+                      double ell, double sf2, const double* alpha, double* mean);
+
+/* 8-lag window features of a device series t of N values: X[i][k] =
+ * t[i + k - 7], zero before t_0 (X device, row-major N x 8), as
+ * MatrixGenerator::readInputVector and the feature loops build them
+ * (MatrixGenerator.cpp:77-99, 196-206, 257-272). */
+int hs_window_features_f64(hs_handle_t h, const double* t, int64_t N, double* X);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,11 @@
  *  - or_cg: closed-form solution of a dense solve (numpy) and the iteration
  *    count of a textbook CG written independently in the tests;
  *  - or_rbf / or_rbf_window: numpy brute force of the textbook RBF formula;
+ *  - or_rbf_cross / or_window_features / or_gemv_gp: numpy brute force of the
+ *    formula and of the window construction, K* = sf2 exactly where a test
+ *    input equals a training input, numpy's dot for the mat-vec, and the GP
+ *    identity that the predictive mean at the training inputs equals
+ *    y - sn2 alpha for alpha = K^{-1} y;
  *  - synthetic generator: an independent numpy SplitMix64 in the tests.
  */
 #include <math.h>
@@ -431,6 +436,49 @@ void or_rbf(const double* X, i64 N, int d, int b, double ell, double sf2, double
                 }
             }
         }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * GP prediction. Reference: MatrixGenerator::generateTestKernelMatrix,
+ * src/utility/MatrixGenerator.cpp:163-213 (features: the 8-lag windows that
+ * readInputVector, :257-272, builds with 7 leading zeros; per entry the
+ * distance is summed in lag order as tmp = train - test, then
+ * vl * exp(-0.5 / (l * l) * dist), stored transposed at K*[j N + i]), and
+ * MatrixVectorOperations::matrixVectorGP, src/kernel/MatrixVectorOperations.cpp:
+ * 379-395 (one work item per row, k ascending). or_rbf_cross takes general
+ * row-major features and (ell, sf2) with the same per-entry operation order;
+ * the reference fixes vl = l = 1.
+ * ------------------------------------------------------------------------- */
+void or_window_features(const double* t, i64 N, double* X) {
+    const int lags = 8;
+    for (i64 i = 0; i < N; ++i)
+        for (int k = 0; k < lags; ++k) {
+            const i64 src = i + k - (lags - 1);
+            X[i * lags + k] = src >= 0 ? t[src] : 0.0;
+        }
+}
+
+void or_rbf_cross(const double* Xtr, i64 N, const double* Xte, i64 M, int d, double ell, double sf2, double* K) {
+#pragma omp parallel for schedule(static)
+    for (i64 j = 0; j < M; ++j) {
+        for (i64 i = 0; i < N; ++i) {
+            double dist = 0.0;
+            for (int k = 0; k < d; ++k) {
+                const double tmp = Xtr[i * d + k] - Xte[j * d + k];
+                dist += tmp * tmp;
+            }
+            K[j * N + i] = sf2 * exp(-0.5 / (ell * ell) * dist);
+        }
+    }
+}
+
+void or_gemv_gp(const double* A, const double* b, double* result, i64 n, i64 m) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < m; ++i) {
+        double sum = 0.0;
+        for (i64 k = 0; k < n; ++k) sum += A[i * n + k] * b[k];
+        result[i] = sum;
     }
 }
 

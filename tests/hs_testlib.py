@@ -63,6 +63,9 @@ class Oracle:
             "or_rbf_window": ([_P, _L, _I, _P], None),
             "or_rbf": ([_P, _L, _I, _I, _D, _D, _D, _P], None),
             "or_synth_gp": ([C.c_uint64, _L, _I, _P, _P], None),
+            "or_window_features": ([_P, _L, _P], None),
+            "or_rbf_cross": ([_P, _L, _P, _L, _I, _D, _D, _P], None),
+            "or_gemv_gp": ([_P, _P, _P, _L, _L], None),
             "or_num_threads": ([], _I),
         }
         for name, (args, res) in sig.items():
@@ -100,6 +103,28 @@ class Oracle:
         A = np.zeros(packed_len(N, b))
         self.L.or_rbf(dptr(np.ascontiguousarray(X)), N, d, b, ell, sf2, sn2, dptr(A))
         return A
+
+    def rbf_cross(self, Xtr, Xte, ell, sf2):
+        Xtr, Xte = np.ascontiguousarray(Xtr, dtype=np.float64), np.ascontiguousarray(Xte, dtype=np.float64)
+        N, d = Xtr.shape
+        M = Xte.shape[0]
+        K = np.empty(M * N)
+        self.L.or_rbf_cross(dptr(Xtr), N, dptr(Xte), M, d, ell, sf2, dptr(K))
+        return K.reshape(M, N)
+
+    def gemv_gp(self, A, x):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        m, n = A.shape
+        y = np.empty(m)
+        self.L.or_gemv_gp(dptr(A), dptr(x), dptr(y), n, m)
+        return y
+
+    def window(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        X = np.empty(len(t) * 8)
+        self.L.or_window_features(dptr(t), len(t), dptr(X))
+        return X.reshape(len(t), 8)
 
     def synth(self, seed, N, d):
         X = np.empty(N * d)

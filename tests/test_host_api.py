@@ -63,3 +63,10 @@ def test_cg_rejects_mismatched_dimensions():
     out = run("cgdims")
     assert out[0].startswith("invalid_argument Dimensions of A and b do not match: 20 != 21")
     assert out[1].startswith("invalid_argument Block count of A and b do not match")
+
+
+def test_gp_matvec_rejects_cpu_queue():
+    """The GP mat-vec computes on a GPU queue only: a CPU queue is an error,
+    not a silent host fallback."""
+    out = run("gpcpu")
+    assert out[0].startswith("invalid_argument matrixVectorGP")

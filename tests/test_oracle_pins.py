@@ -131,3 +131,63 @@ def test_packed_layout_ids(oracle):
     # column-major enumeration: block (J,J) starts column J
     for J in range(n):
         assert ids[J][J] == sum(n - c for c in range(J))
+
+
+# ---- GP prediction (K* assembly, window features, the GP mat-vec) ----------
+
+def _cross_numpy(Xtr, Xte, ell, sf2):
+    d2 = ((Xte[:, None, :] - Xtr[None, :, :]) ** 2).sum(-1)
+    return sf2 * np.exp(-0.5 / ell ** 2 * d2)
+
+
+@pytest.mark.parametrize("N,M,d,ell,sf2", [(40, 7, 3, 0.8, 1.7), (64, 33, 8, 1.0, 1.0), (5, 1, 16, 2.0, 0.5),
+                                           (1, 4, 1, 0.3, 2.0)])
+def test_rbf_cross_matches_numpy(oracle, N, M, d, ell, sf2):
+    rng = np.random.default_rng(N * 100 + M)
+    Xtr, Xte = rng.uniform(-1, 1, (N, d)), rng.uniform(-1, 1, (M, d))
+    K = oracle.rbf_cross(Xtr, Xte, ell, sf2)
+    assert K.shape == (M, N)
+    assert np.allclose(K, _cross_numpy(Xtr, Xte, ell, sf2), rtol=1e-14, atol=1e-300)
+
+
+def test_rbf_cross_equal_inputs_give_sf2(oracle):
+    """A test input equal to training input i gives K*[j, i] = sf2 exactly
+    (distance 0, exp(0) = 1); a reversed layout or a dropped transpose moves
+    the ones off the diagonal."""
+    rng = np.random.default_rng(5)
+    Xtr = rng.uniform(-1, 1, (20, 8))
+    K = oracle.rbf_cross(Xtr, Xtr[[3, 11, 19]], 1.3, 2.5)
+    for j, i in enumerate([3, 11, 19]):
+        assert K[j, i] == 2.5
+        assert np.all(np.delete(K[j], i) < 2.5)
+
+
+def test_window_features_match_reference_construction(oracle):
+    """readInputVector puts the series after 7 zeros and the features of
+    point i are entries i..i+7 of that padded vector."""
+    t = np.arange(1.0, 13.0)
+    X = oracle.window(t)
+    tin = np.concatenate([np.zeros(7), t])
+    assert np.array_equal(X, np.stack([tin[i:i + 8] for i in range(len(t))]))
+    assert np.array_equal(X[0], [0, 0, 0, 0, 0, 0, 0, 1.0])
+    assert np.array_equal(X[9], np.arange(3.0, 11.0))
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (7, 300), (33, 64)])
+def test_gemv_gp_matches_numpy(oracle, m, n):
+    rng = np.random.default_rng(m + n)
+    A, x = rng.standard_normal((m, n)), rng.standard_normal(n)
+    assert np.allclose(oracle.gemv_gp(A, x), A @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_prediction_at_training_inputs_is_y_minus_noise_alpha(oracle):
+    """K*(X, X) = K - sn2 I, so the predictive mean at the training inputs is
+    K* alpha = y - sn2 alpha for alpha = K^{-1} y: ties the K* assembly, the
+    mat-vec and the solve together through a closed form."""
+    N, d, ell, sf2, sn2 = 120, 4, 0.9, 1.2, 0.05
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, (N, d))
+    y = np.sin(X.sum(1))
+    alpha = np.linalg.solve(rbf_dense_numpy(X, ell, sf2, sn2), y)
+    mean = oracle.gemv_gp(oracle.rbf_cross(X, X, ell, sf2), alpha)
+    assert np.allclose(mean, y - sn2 * alpha, rtol=1e-9, atol=1e-9)
