@@ -300,6 +300,55 @@ def adjoint_workload(hs, dev, syn, args):
             "flops": N ** 3 / 3.0 + bwd_flops, "tflops": (N ** 3 / 3.0 + bwd_flops) / (ms / 1e3) / 1e12}
 
 
+def gpr_workload(hs, dev, syn, args):
+    """GP prediction on the CG workload's training inputs (N = 32768, d = 8,
+    ell = 1, sf2 = 1) at M = 8192 test inputs (seed 102): the predictive mean
+    K* alpha, fused (hs_gp_predict_f64: K* is never stored) and as the
+    reference's two steps (hs_rbf_cross_f64 writes K*, hs_gemv_rows_f64 reads
+    it). alpha is the training target vector: the cost does not depend on it."""
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cg"]
+    M = 8192
+    X, y = syn.gp_inputs(seed, N, d)
+    Xt, _ = syn.gp_inputs(seed + 100, M, d)
+    dX, dXt, da = dev.upload(X.reshape(-1)), dev.upload(Xt.reshape(-1)), dev.upload(y)
+    dm, dm2, dK = dev.zeros(M), dev.zeros(M), dev.alloc(N * M * 8)
+    timer = hs.Timer(dev)
+    reps = 5
+
+    def fused():
+        hs.call("hs_gp_predict_f64", dev.h, dX.p, N, dXt.p, M, d, ell, sf2, da.p, dm.p)
+
+    def assemble():
+        hs.call("hs_rbf_cross_f64", dev.h, dX.p, N, dXt.p, M, d, ell, sf2, dK.p)
+
+    def gemv():
+        hs.call("hs_gemv_rows_f64", dev.h, dK.p, da.p, dm2.p, N, M)
+
+    out = {"N": N, "M": M, "d": d, "ell": ell, "sf2": sf2}
+    for name, f in (("fused", fused), ("kstar_assembly", assemble), ("kstar_gemv", gemv)):
+        f()
+        timer.start()
+        for _ in range(reps):
+            f()
+        timer.stop()
+        out[name + "_ms"] = timer.ms() / reps
+    timer.close()
+    a, c = dev.download(dm, M), dev.download(dm2, M)
+    dK.free()
+    pairs = float(N) * M
+    kbytes = pairs * 8
+    out["pairs_per_s_fused"] = pairs / (out["fused_ms"] / 1e3)
+    out["two_step_ms"] = out["kstar_assembly_ms"] + out["kstar_gemv_ms"]
+    out["kstar_assembly_gbs"] = kbytes / (out["kstar_assembly_ms"] / 1e3) / 1e9
+    out["kstar_gemv_gbs"] = kbytes / (out["kstar_gemv_ms"] / 1e3) / 1e9
+    out["fused_vs_two_step_max_rel_diff"] = float(np.max(np.abs(a - c)) / np.max(np.abs(c)))
+    log("gpr workload N=%d M=%d d=%d: fused %.2f ms (%.3g pairs/s); two steps %.2f ms (K* %.2f ms = %.0f GB/s, "
+        "mat-vec %.2f ms = %.0f GB/s); max rel diff %.1e" %
+        (N, M, d, out["fused_ms"], out["pairs_per_s_fused"], out["two_step_ms"], out["kstar_assembly_ms"],
+         out["kstar_assembly_gbs"], out["kstar_gemv_ms"], out["kstar_gemv_gbs"], out["fused_vs_two_step_max_rel_diff"]))
+    return out
+
+
 def cpu_baseline(args):
     """The oracle (C restatement of the reference's CPU path, OpenMP) on a
     bounded sample of the same workload: the N = 8192 system of the same
@@ -336,7 +385,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=8192)
-    ap.add_argument("--only", choices=["cg", "symv", "adjoint"],
+    ap.add_argument("--no-gpr", action="store_true")
+    ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr"],
                     help="run only this side workload and print its record (for profiling; not a bench line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a kernel of known HBM traffic (y = 2x over 2^27 doubles, 5 launches) to scale "
@@ -363,6 +413,8 @@ def main():
         with hs.Device(0) as dev:
             if args.only == "adjoint":
                 rec = adjoint_workload(hs, dev, syn, args)
+            elif args.only == "gpr":
+                rec = gpr_workload(hs, dev, syn, args)
             else:
                 rec = cg_workload(hs, dev, syn, args, symv_only=(args.only == "symv"))
         print(json.dumps({args.only: rec}))
@@ -372,6 +424,7 @@ def main():
         mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args)
         cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
         adj = None if args.no_adjoint else adjoint_workload(hs, dev, syn, args)
+        gpr = None if args.no_gpr else gpr_workload(hs, dev, syn, args)
     if mixed:
         fp64_solve = ch["factor_ms"] + ch["solve_ms"]
         for v in mixed.values():
@@ -406,6 +459,7 @@ def main():
         "mixed": mixed,
         "cg": cg,
         "adjoint": adj,
+        "gpr": gpr,
     }
     print(json.dumps(out))
 

@@ -26,17 +26,24 @@ __device__ inline void decode_block_col(i64 id, int n, int* I, int* J) {
 }
 
 // Fast path: b in {64, 128, 256}, feature count D fixed at compile time.
-// Thread t owns column j = t % b (its features in registers) and rows
-// i = t / b + s * (256 / b); x_i is read through uniform (scalar) loads.
+// The block row's features (b x D, contiguous in X) are staged in LDS once;
+// thread t owns column j = t % b (its features in registers) and rows
+// i = t / b + s * (256 / b), and all lanes of a wave share the row, so x_i
+// comes from LDS as a broadcast. Each row of the block is one coalesced run
+// of non-temporal stores (the matrix is written once here and read later).
 template <int BS, int D>
 __global__ __launch_bounds__(RBF_THREADS) void rbf_fast_kernel(const double* __restrict__ X, i64 N, int n,
                                                                 double scale, double sf2, double sn2,
                                                                 double* __restrict__ A) {
     constexpr int RP = RBF_THREADS / BS;  // row phases
+    __shared__ double xs[BS * D];
     const i64 id = blockIdx.x;
     int I, J;
     decode_block_col(id, n, &I, &J);
     double* blk = A + id * (i64)BS * BS;
+    const i64 rowBase = (i64)I * BS;
+    const i64 rowValid = std::min<i64>((i64)BS, N - rowBase);
+    for (int e = threadIdx.x; e < BS * D; e += RBF_THREADS) xs[e] = (e / D < rowValid) ? X[rowBase * D + e] : 0.0;
     const int j = threadIdx.x % BS;
     const int ph = threadIdx.x / BS;
     const i64 jg = (i64)J * BS + j;
@@ -44,12 +51,12 @@ __global__ __launch_bounds__(RBF_THREADS) void rbf_fast_kernel(const double* __r
     double xj[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) xj[k] = jvalid ? X[jg * D + k] : 0.0;
-    const i64 rowValid = std::min<i64>((i64)BS, N - (i64)I * BS);
+    __syncthreads();
     for (int i = ph; i < BS; i += RP) {
-        const i64 ig = (i64)I * BS + i;
+        const i64 ig = rowBase + i;
         double v = 0.0;
         if (i < rowValid && jvalid) {
-            const double* xi = X + ig * D;
+            const double* xi = xs + i * D;
             double dist = 0.0;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
@@ -59,7 +66,7 @@ __global__ __launch_bounds__(RBF_THREADS) void rbf_fast_kernel(const double* __r
             v = sf2 * exp(scale * dist);
             if (ig == jg) v += sn2;
         }
-        blk[(i64)i * BS + j] = v;
+        __builtin_nontemporal_store(v, blk + (i64)i * BS + j);
     }
 }
 
