@@ -351,8 +351,9 @@ def gpr_workload(hs, dev, syn, args):
 
 def cpu_baseline(args):
     """The oracle (C restatement of the reference's CPU path, OpenMP) on a
-    bounded sample of the same workload: the N = 8192 system of the same
-    construction (d = 16, b = 256, l = 2, noise 1e-3)."""
+    bounded sample of the same workload: the N = 14336 system of the same
+    construction (d = 16, b = 256, l = 2, noise 1e-3), about 12 s of work on
+    16 host threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from hs_testlib import Oracle
 
@@ -384,7 +385,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-n", type=int, default=14336)
     ap.add_argument("--no-gpr", action="store_true")
     ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr"],
                     help="run only this side workload and print its record (for profiling; not a bench line)")
