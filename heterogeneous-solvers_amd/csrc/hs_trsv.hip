@@ -193,6 +193,38 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_cols_update_kernel(const T* _
     }
 }
 
+// Column-slice form of the same update for the steps with few blocks: work-
+// group (I, slice) owns CW = 64 columns of block (j, I), lane l column
+// slice * 64 + l, wave w a quarter of the rows (one 8-byte load per lane and
+// row, eight rows in flight); the four wave sums are added in wave order.
+// Four work-groups per block instead of one keep more CUs streaming while the
+// block row j is short.
+template <typename T, int BS>
+__global__ __launch_bounds__(TS_THREADS) void gemv_cols_slice_kernel(const T* __restrict__ L,
+                                                                     const double* __restrict__ v,
+                                                                     double* __restrict__ z, int n, int j) {
+    constexpr int CW = 64;
+    constexpr int SL = BS / CW;  // slices per block
+    constexpr int RQ = BS / 4;   // rows per wave
+    __shared__ double xs[BS];
+    __shared__ double red[4][CW];
+    const int I = blockIdx.x / SL;
+    const int c0 = (blockIdx.x % SL) * CW;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < BS; i += TS_THREADS) xs[i] = v[(i64)j * BS + i];
+    __syncthreads();
+    const T* col = L + hs_block_id(j, I, n) * (i64)BS * BS + c0 + lane;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < RQ; ++k) {
+        const int r = wave * RQ + k;
+        acc += ld(col + (i64)r * BS) * xs[r];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (t < CW) z[(i64)I * BS + c0 + t] -= (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+}
+
 }  // namespace
 
 bool hs_trsv_fast_ok(int b) { return b == 64 || b == 128 || b == 256; }
@@ -243,8 +275,12 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
             Dup + (i64)j * BS * BS, z + (i64)j * BS, v + (i64)j * BS, strips, N - (i64)j * BS);
         HS_LAUNCH_CHECK();
         if (j > 0) {
-            // z_I -= L_jI^T x_j for blocks (j, I), I < j
-            gemv_cols_update_kernel<T, BS><<<dim3((unsigned)j), blk, 0, h->stream>>>(L, v, z, n, j);
+            // z_I -= L_jI^T x_j for blocks (j, I), I < j; column slices while
+            // the row is shorter than the CU count
+            if (BS >= 128 && j < h->cu_count)
+                gemv_cols_slice_kernel<T, BS><<<dim3((unsigned)(j * (BS / 64))), blk, 0, h->stream>>>(L, v, z, n, j);
+            else
+                gemv_cols_update_kernel<T, BS><<<dim3((unsigned)j), blk, 0, h->stream>>>(L, v, z, n, j);
             HS_LAUNCH_CHECK();
         }
     }
