@@ -387,7 +387,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=14336)
     ap.add_argument("--no-gpr", action="store_true")
-    ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr"],
+    ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr", "mixed"],
                     help="run only this side workload and print its record (for profiling; not a bench line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a kernel of known HBM traffic (y = 2x over 2^27 doubles, 5 launches) to scale "
@@ -416,6 +416,8 @@ def main():
                 rec = adjoint_workload(hs, dev, syn, args)
             elif args.only == "gpr":
                 rec = gpr_workload(hs, dev, syn, args)
+            elif args.only == "mixed":
+                rec = mixed_workload(hs, dev, syn, args)
             else:
                 rec = cg_workload(hs, dev, syn, args, symv_only=(args.only == "symv"))
         print(json.dumps({args.only: rec}))

@@ -24,6 +24,7 @@ namespace {
 constexpr int THREADS = 256;
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // K (FP64 packed) -> FP32 packed copy, + shift on the global diagonal
 __global__ void to_f32_kernel(const double* __restrict__ K, float* __restrict__ L, i64 total) {
@@ -187,8 +188,17 @@ __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((u
 // planes row-major [row][k] with an 80-byte row pitch, so the 16-byte
 // fragment reads of 16 lanes (8 consecutive k of 16 rows) hit distinct banks.
 // Fragment of lane l: row l%16, k = 8(l/16)..+7; output as the f32 MFMA.
-template <int TILE, int KIND>
-__global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restrict__ A, int b, int n, PairMap pm) {
+//
+// PS (pre-split): the operand blocks come from hi / lo planes that
+// split_panels_bf16_kernel wrote once for the whole launch (Wh, Wl: np
+// panels x m block rows from rs, b x b bf16 each), so the stage fill is a
+// copy; otherwise every work-group splits the FP32 values it loads, which
+// costs more VALU issue than the MFMAs of the stage.
+template <int TILE, int KIND, bool PS>
+__global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                 const unsigned short* __restrict__ Wh,
+                                                                 const unsigned short* __restrict__ Wl, int rs,
+                                                                 int m) {
     constexpr int KT = 32;
     constexpr int WT = TILE / 2, MT = WT / 16;
     constexpr int P = 40;                 // row pitch in bf16 elements (80 bytes)
@@ -210,9 +220,13 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     constexpr int LPT = TILE * KT / THREADS;  // 16 floats per thread (TILE 128)
     constexpr int TPR = KT / LPT;             // 2 threads per row
     const int lrow = tid / TPR, lk = (tid % TPR) * LPT;
-    float4 ra[LPT / 4], rb[LPT / 4];
+    float4 ra[PS ? 1 : LPT / 4], rb[PS ? 1 : LPT / 4];
+    u32x4 rah[PS ? LPT / 8 : 1], ral[PS ? LPT / 8 : 1], rbh[PS ? LPT / 8 : 1], rbl[PS ? LPT / 8 : 1];
     const float* pa = La + (i64)lrow * b + lk;
     const float* pb = Lb + (i64)lrow * b + lk;
+    // plane offsets of this thread's rows in block rows I and J of panel 0
+    const i64 wa = PS ? ((i64)(I - rs) * bb + (i64)(tm * TILE + lrow) * b + lk) : 0;
+    const i64 wb = PS ? ((i64)(J - rs) * bb + (i64)(tn * TILE + lrow) * b + lk) : 0;
     f4 acc[MT][MT];
 #pragma unroll
     for (int i = 0; i < MT; ++i)
@@ -226,10 +240,21 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
         return (pp * (n - pm.k - 1) - pp * (pp - 1) / 2) * bb + (i64)(kt % nk) * KT;
     };
     auto gload = [&](int kt) {
+        if constexpr (PS) {
+            const i64 o = (i64)(kt / nk) * m * bb + (i64)(kt % nk) * KT;
 #pragma unroll
-        for (int v = 0; v < LPT / 4; ++v) {
-            ra[v] = *reinterpret_cast<const float4*>(pa + poff(kt) + 4 * v);
-            rb[v] = *reinterpret_cast<const float4*>(pb + poff(kt) + 4 * v);
+            for (int v = 0; v < LPT / 8; ++v) {
+                rah[v] = *reinterpret_cast<const u32x4*>(Wh + wa + o + 8 * v);
+                ral[v] = *reinterpret_cast<const u32x4*>(Wl + wa + o + 8 * v);
+                rbh[v] = *reinterpret_cast<const u32x4*>(Wh + wb + o + 8 * v);
+                rbl[v] = *reinterpret_cast<const u32x4*>(Wl + wb + o + 8 * v);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < LPT / 4; ++v) {
+                ra[v] = *reinterpret_cast<const float4*>(pa + poff(kt) + 4 * v);
+                rb[v] = *reinterpret_cast<const float4*>(pb + poff(kt) + 4 * v);
+            }
         }
     };
     auto split_store = [&](const float4* r, unsigned short* hiPlane, unsigned short* loPlane) {
@@ -263,8 +288,19 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     };
     auto sstore = [&](int buf) {
         unsigned short* s = lds + buf * STAGE;
-        split_store(ra, s, s + PLANE);
-        split_store(rb, s + 2 * PLANE, s + 3 * PLANE);
+        if constexpr (PS) {
+            u32x4* d = reinterpret_cast<u32x4*>(s + lrow * P + lk);
+#pragma unroll
+            for (int v = 0; v < LPT / 8; ++v) {
+                d[v] = rah[v];
+                d[PLANE / 8 + v] = ral[v];
+                d[2 * PLANE / 8 + v] = rbh[v];
+                d[3 * PLANE / 8 + v] = rbl[v];
+            }
+        } else {
+            split_store(ra, s, s + PLANE);
+            split_store(rb, s + 2 * PLANE, s + 3 * PLANE);
+        }
     };
     gload(0);
     sstore(0);
@@ -305,10 +341,39 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
     epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
+// The bf16 hi / lo planes of np panels (block columns k .. k+np-1) for the
+// m block rows from rs: W[(p m + i) b^2 + e] for block (rs + i, k + p),
+// hi = rn(x), lo = rn(x - hi), eight values per thread.
+__global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __restrict__ L, int b, int n, int k,
+                                                                int np, int rs, int m, unsigned short* __restrict__ Wh,
+                                                                unsigned short* __restrict__ Wl) {
+    const i64 bb = (i64)b * b;
+    const i64 total = (i64)np * m * bb;
+    for (i64 e = ((i64)blockIdx.x * blockDim.x + threadIdx.x) * 8; e < total; e += (i64)gridDim.x * blockDim.x * 8) {
+        const i64 blk = e / bb, off = e % bb;
+        const int p = (int)(blk / m), i = (int)(blk % m);
+        const float* src = L + hs_block_id(rs + i, k + p, n) * bb + off;
+        const float4 x0 = *reinterpret_cast<const float4*>(src);
+        const float4 x1 = *reinterpret_cast<const float4*>(src + 4);
+        const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        unsigned hi[4], lo[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const unsigned short h0 = bf16_rn(x[2 * c]), h1 = bf16_rn(x[2 * c + 1]);
+            const unsigned short l0 = bf16_rn(x[2 * c] - bf16_to_f(h0));
+            const unsigned short l1 = bf16_rn(x[2 * c + 1] - bf16_to_f(h1));
+            hi[c] = h0 | ((unsigned)h1 << 16);
+            lo[c] = l0 | ((unsigned)l1 << 16);
+        }
+        *reinterpret_cast<uint4*>(Wh + e) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+        *reinterpret_cast<uint4*>(Wl + e) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    }
+}
+
 }  // namespace
 
 int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
-                             int n, int mode, int variant, int np) {
+                             int n, int mode, int variant, int np, unsigned short* planes) {
     if (rowCount <= 0) return HS_OK;
     if (b % 128 != 0) {
         hs_set_error("mixed-precision trailing update needs b % 128 == 0");
@@ -325,8 +390,21 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     }
     const dim3 g((unsigned)grid), t(THREADS);
     if (variant == HS_MIXED_BF16X3) {
-        if (mode >= 5) trailing_mfma_bf16x3<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
-        else trailing_mfma_bf16x3<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode == 4 && planes) {
+            // rows and columns of a mode-4 launch start at rs = k + 2 np
+            const int rs = k + 2 * np, m = n - rs;
+            const i64 half = (i64)np * m * b * b;
+            unsigned short* Wh = planes;
+            unsigned short* Wl = planes + half;
+            const i64 sg = std::min<i64>(8192, (half / 8 + 255) / 256);
+            split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(A, b, n, k, np, rs, m, Wh, Wl);
+            HS_LAUNCH_CHECK();
+            trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, rs, m);
+        } else if (mode >= 5) {
+            trailing_mfma_bf16x3<TILE, 1, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
+        } else {
+            trailing_mfma_bf16x3<TILE, 0, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
+        }
     } else {
         if (mode >= 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f32<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
@@ -356,12 +434,19 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
+    // bf16 hi / lo planes of one rest update's panels: at most 4 panels of
+    // n block rows, 4 bytes per element in all
+    unsigned short* planes = nullptr;
+    if (variant == HS_MIXED_BF16X3) {
+        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)4 * n * b * b))) return rc;
+        planes = (unsigned short*)h->work;
+    }
     auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
     auto trsm = [&](int k, int start, int count) {
         return hs_factor_trsm<float>(h, L32, b, hs_block_id(k, k, n), k, start, count);
     };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
-        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np);
+        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np, planes);
     };
     if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
         hipStreamSynchronize(h->stream2);
