@@ -49,33 +49,33 @@ __global__ void diag_shift_kernel(const double* __restrict__ K, float* __restric
 // stores: the compiler cannot prove the stores do not alias later loads and
 // would otherwise wait for every load on its own. On the diagonal tile of a
 // diagonal block only the lower triangle is written.
-template <int MT>
+template <int MI, int MJ>
 __device__ inline void epilogue_f32(float* C, int b, int wm0, int wn0, int lr, int lq, bool dtile,
-                                    const f4 (&acc)[MT][MT]) {
+                                    const f4 (&acc)[MI][MJ]) {
     float* pc = C + (i64)(wm0 + 4 * lq) * b + wn0 + lr;
-    float cv[MT][4], cn[MT][4];
+    float cv[MJ][4], cn[MJ][4];
 #pragma unroll
-    for (int j = 0; j < MT; ++j)
+    for (int j = 0; j < MJ; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) cv[j][v] = pc[(i64)v * b + j * 16];
 #pragma unroll
-    for (int i = 0; i < MT; ++i) {
-        if (i + 1 < MT) {
+    for (int i = 0; i < MI; ++i) {
+        if (i + 1 < MI) {
 #pragma unroll
-            for (int j = 0; j < MT; ++j)
+            for (int j = 0; j < MJ; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) cn[j][v] = pc[(i64)((i + 1) * 16 + v) * b + j * 16];
         }
 #pragma unroll
-        for (int j = 0; j < MT; ++j)
+        for (int j = 0; j < MJ; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = wm0 + i * 16 + 4 * lq + v, col = wn0 + j * 16 + lr;
                 if (!dtile || row >= col) pc[(i64)(i * 16 + v) * b + j * 16] = cv[j][v] - acc[i][j][v];
             }
-        if (i + 1 < MT) {
+        if (i + 1 < MI) {
 #pragma unroll
-            for (int j = 0; j < MT; ++j)
+            for (int j = 0; j < MJ; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) cv[j][v] = cn[j][v];
         }
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
         }
         __syncthreads();
     }
-    epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+    epilogue_f32<MT, MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
 // ---- BF16 hi+lo split trailing update -------------------------------------
@@ -338,7 +338,108 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
             }
         __syncthreads();
     }
-    epilogue_f32<MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+    epilogue_f32<MT, MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+}
+
+// Rest update from pre-split planes in 256 x 256 tiles (b % 256 == 0): 512
+// threads as 4 x 2 waves of 64 x 128, KT = 32 stages of the four planes
+// (80 KB each, double-buffered in 160 KB of dynamic LDS), the B fragments
+// taken in two halves of 64 columns to stay within 256 VGPRs. A quarter less
+// LDS read traffic and half the operand traffic per MFMA of the 128 tile.
+constexpr int T2 = 256, T2THREADS = 512, T2P = 40;
+constexpr int T2PLANE = T2 * T2P;                   // bf16 elements per plane
+constexpr int T2STAGE = 4 * T2PLANE;
+constexpr size_t T2LDS = sizeof(unsigned short) * 2 * T2STAGE;
+
+__global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                   const unsigned short* __restrict__ Wh,
+                                                                   const unsigned short* __restrict__ Wl, int rs,
+                                                                   int m) {
+    constexpr int KT = 32;
+    extern __shared__ unsigned short lds2[];
+    const int tpe = b / T2, tpb = tpe * tpe;
+    int I, J, tile;
+    if (!tile_of(pm, tpb, &I, &J, &tile)) return;
+    const int tm = tile / tpe, tn = tile % tpe;
+    const bool diag = (I == J);
+    if (diag && tm < tn) return;
+    const i64 bb = (i64)b * b;
+    float* C = A + hs_block_id(I, J, n) * bb + (i64)tm * T2 * b + (i64)tn * T2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 128;
+    // staging: thread -> row tid / 2, 16 consecutive k from (tid & 1) * 16
+    const int lrow = tid >> 1, lk = (tid & 1) * 16;
+    const i64 wa = (i64)(I - rs) * bb + (i64)(tm * T2 + lrow) * b + lk;
+    const i64 wb = (i64)(J - rs) * bb + (i64)(tn * T2 + lrow) * b + lk;
+    u32x4 rah[2], ral[2], rbh[2], rbl[2];
+    f4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const int nk = b / KT, nkt = nk * pm.np;
+    auto gload = [&](int kt) {
+        const i64 o = (i64)(kt / nk) * m * bb + (i64)(kt % nk) * KT;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            rah[v] = *reinterpret_cast<const u32x4*>(Wh + wa + o + 8 * v);
+            ral[v] = *reinterpret_cast<const u32x4*>(Wl + wa + o + 8 * v);
+            rbh[v] = *reinterpret_cast<const u32x4*>(Wh + wb + o + 8 * v);
+            rbl[v] = *reinterpret_cast<const u32x4*>(Wl + wb + o + 8 * v);
+        }
+    };
+    auto sstore = [&](int buf) {
+        u32x4* d = reinterpret_cast<u32x4*>(lds2 + buf * T2STAGE + lrow * T2P + lk);
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            d[v] = rah[v];
+            d[T2PLANE / 8 + v] = ral[v];
+            d[2 * T2PLANE / 8 + v] = rbh[v];
+            d[3 * T2PLANE / 8 + v] = rbl[v];
+        }
+    };
+    gload(0);
+    sstore(0);
+    if (nkt > 1) gload(1);
+    __syncthreads();
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int kt = 0; kt < nkt; ++kt) {
+        const unsigned short* s = lds2 + (kt & 1) * T2STAGE;
+        bf8 ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int off = (wm0 + i * 16 + lr) * T2P + 8 * lq;
+            ah[i] = *reinterpret_cast<const bf8*>(s + off);
+            al[i] = *reinterpret_cast<const bf8*>(s + T2PLANE + off);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf8 bh[4], bl[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int off = (wn0 + (4 * h + j) * 16 + lr) * T2P + 8 * lq;
+                bh[j] = *reinterpret_cast<const bf8*>(s + 2 * T2PLANE + off);
+                bl[j] = *reinterpret_cast<const bf8*>(s + 3 * T2PLANE + off);
+            }
+            // the other buffer was last read in stage kt-1: fill it while
+            // this stage's MFMAs run
+            if (h == 0 && kt + 1 < nkt) {
+                sstore((kt + 1) & 1);
+                if (kt + 2 < nkt) gload(kt + 2);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    f4& c = acc[i][4 * h + j];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
 // The bf16 hi / lo planes of np panels (block columns k .. k+np-1) for the
@@ -399,7 +500,21 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
             const i64 sg = std::min<i64>(8192, (half / 8 + 255) / 256);
             split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(A, b, n, k, np, rs, m, Wh, Wl);
             HS_LAUNCH_CHECK();
-            trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, rs, m);
+            if (b % T2 == 0) {
+                static bool attr = false;
+                if (!attr) {
+                    HS_HIP(hipFuncSetAttribute((const void*)trailing_bf16x3_ps256,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)T2LDS));
+                    attr = true;
+                }
+                PairMap pm2;
+                const i64 grid2 = hs_pair_map(k, rowStart, rowCount, n, mode, np, (i64)(b / T2) * (b / T2), &pm2);
+                if (grid2 > 0)
+                    trailing_bf16x3_ps256<<<dim3((unsigned)grid2), dim3(T2THREADS), T2LDS, stream>>>(A, b, n, pm2, Wh,
+                                                                                                   Wl, rs, m);
+            } else {
+                trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, rs, m);
+            }
         } else if (mode >= 5) {
             trailing_mfma_bf16x3<TILE, 1, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
         } else {
