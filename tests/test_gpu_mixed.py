@@ -23,7 +23,8 @@ def system(oracle, N, d, b, ell, sn2, seed):
 
 
 @pytest.mark.parametrize("variant", [0, 1])
-@pytest.mark.parametrize("N,b", [(1024, 128), (1536, 256), (1280, 256), (600, 128)])
+@pytest.mark.parametrize("N,b", [(1024, 128), (1536, 256), (1280, 256), (600, 128), (4096, 256), (3000, 256),
+                                 (4096, 128)])
 def test_mixed_factor_reconstructs(hs, dev, oracle, variant, N, b):
     print("mixed factor variant=%d N=%d b=%d" % (variant, N, b), flush=True)
     A, _ = system(oracle, N, 8, b, 1.0, 1e-1, 3)
