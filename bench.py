@@ -378,7 +378,7 @@ def cpu_baseline(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=0, help="override N of the Cholesky workload (testing only)")
     ap.add_argument("--no-cg", action="store_true")
