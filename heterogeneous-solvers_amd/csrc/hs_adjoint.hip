@@ -521,7 +521,7 @@ AdjWork adj_layout(int n, int b) {
         o += (cnt + 31) / 32 * 32;
         return at;
     };
-    w.dinv = take(2 * (i64)n * bb);
+    w.dinv = take(hs_dinv_doubles(n, b));
     w.v = take(nb);
     w.z = take(nb);
     w.zs = take(nb);

@@ -303,10 +303,11 @@ extern "C" int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64
     const int n = (int)((N + b - 1) / b);
     int rc;
     if (hs_trsv_fast_ok(b)) {
-        // workspace: L_jj^{-T} and L_jj^{-1} blocks, then the forward vector
-        if ((rc = hs_ensure_work(h, sizeof(double) * ((size_t)2 * n * b * b + (size_t)n * b)))) return rc;
+        // workspace: the inverted diagonal (super-)blocks, then the forward vector
+        const size_t nd = (size_t)hs_dinv_doubles(n, b);
+        if ((rc = hs_ensure_work(h, sizeof(double) * (nd + (size_t)n * b)))) return rc;
         double* Dinv = (double*)h->work;
-        double* z = Dinv + (size_t)2 * n * b * b;
+        double* z = Dinv + nd;
         if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
         return hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b);
     }
@@ -337,9 +338,10 @@ extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int
     int rc;
     if (hs_trsv_fast_ok(b)) {
         const size_t nb = (size_t)n * b;
-        if ((rc = hs_ensure_work(h, sizeof(double) * ((size_t)2 * n * b * b + nb)))) return rc;
+        const size_t nd = (size_t)hs_dinv_doubles(n, b);
+        if ((rc = hs_ensure_work(h, sizeof(double) * (nd + nb)))) return rc;
         double* Dinv = (double*)h->work;
-        double* z = Dinv + (size_t)2 * n * b * b;
+        double* z = Dinv + nd;
         if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
         if (trans == 0) {
             if ((rc = hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b, 1))) return rc;

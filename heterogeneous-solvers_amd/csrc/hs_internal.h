@@ -93,6 +93,8 @@ template <typename ST>
 int hs_factor_trsm(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
 // fast blocked triangular solves (b in {64, 128, 256}); factor in FP64 or FP32
 bool hs_trsv_fast_ok(int b);
+// doubles of the Dinv workspace of hs_diag_inverse for n blocks of b
+i64 hs_dinv_doubles(int n, int b);
 template <typename T>
 int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv);
 // phases: bit 0 forward (z = L^{-1} v), bit 1 backward (v = L^{-T} z)

@@ -168,12 +168,13 @@ extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32,
     c.rows = (i64)c.n * b;
     const i64 nb = (i64)c.n * (c.n + 1) / 2;
     const i64 G = c.rows / 64 + 2048 + 64;
-    const size_t doubles = (size_t)(2 * c.n * (i64)b * b + 2 * nb * b + 5 * c.rows + G + 16);
+    const i64 nd = hs_dinv_doubles(c.n, b);
+    const size_t doubles = (size_t)(nd + 2 * nb * b + 5 * c.rows + G + 16);
     int rc = hs_ensure_work(h, doubles * sizeof(double));
     if (rc) return rc;
     double* w = (double*)h->work;
     c.Dinv = w;
-    w += 2 * c.n * (i64)b * b;
+    w += nd;
     c.rowpart = w;
     w += nb * b;
     c.colpart = w;

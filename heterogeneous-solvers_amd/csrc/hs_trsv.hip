@@ -106,17 +106,19 @@ __device__ inline double halving_reduce16(double (&v)[R]) {
     return s;
 }
 
-template <typename T, int BS, int SUB>
+// ROWS rows per wave: 16 for the per-block steps; 2 for the 1024-row
+// super-block steps, whose 16 loads per lane and row already keep a wave's
+// loads in flight and which then spread over 128 work-groups.
+template <typename T, int BS, int SUB, int ROWS = 16>
 __global__ __launch_bounds__(TS_THREADS) void gemv_rows_kernel(const T* __restrict__ base, const double* __restrict__ vin,
                                                                double* __restrict__ out, int strips, i64 rowLimit) {
     constexpr int NL = BS / 128 > 0 ? BS / 128 : 1;  // 16-byte loads per lane per row
     constexpr int LPR = BS / 2 < 64 ? BS / 2 : 64;    // lanes per row
     constexpr int RPL = 64 / LPR;                      // rows per wave load
-    constexpr int ROWS = 16;                           // rows per wave
     constexpr int R = ROWS / RPL;                      // row sums per lane
     __shared__ double vs[BS];
     const int g = blockIdx.x / strips;
-    const int r0 = (blockIdx.x % strips) * 64;
+    const int r0 = (blockIdx.x % strips) * (4 * ROWS);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int i = t; i < BS; i += TS_THREADS) vs[i] = vin[i];
     __syncthreads();
@@ -225,17 +227,241 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_cols_slice_kernel(const T* __
     if (t < CW) z[(i64)I * BS + c0 + t] -= (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
+// ---- super-block steps ------------------------------------------------------
+// Every dependent launch of the block substitution costs a launch-to-launch
+// gap of several microseconds, which dominates the solves at n = 256 blocks.
+// The solves therefore step over super-blocks of SB = 1024 rows (k = SB / b
+// blocks): the inverses E_S = L_SS^{-1} of the diagonal super-blocks are
+// formed once per factor (block forward substitution on the block inverses,
+// E_ij = -L_ii^{-1} sum_{m=j}^{i-1} L_im E_mj, one level i - j at a time), and
+// a step is then one SB x SB mat-vec plus one update over the k block
+// columns: 2n / k dependent launches per solve instead of 2n.
+constexpr int SB = 1024;
+
+// 64 x 64 tile of C = sum_t A_t B_t (A row-major of any stored type, B row
+// major double), 256 threads with a 4 x 4 register tile each, 16-deep k
+// stages through LDS.
+template <typename TA>
+__device__ inline void gemm64_acc(const TA* A, i64 lda, const double* B, i64 ldb, int K, double (&acc)[4][4],
+                                  double (*As)[65], double (*Bs)[65]) {
+    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = t + 256 * e;         // 0 .. 1023
+            const int ar = idx >> 4, ak = idx & 15;
+            As[ak][ar] = ld(A + (i64)ar * lda + k0 + ak);
+            const int bk = idx >> 6, bc = idx & 63;
+            Bs[bk][bc] = B[(i64)(k0 + bk) * ldb + bc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double a[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][4 * ty + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][4 * tx + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bv[j];
+        }
+        __syncthreads();
+    }
+}
+
+// E_S <- the block inverses on its diagonal, zero elsewhere
+__global__ void super_init_kernel(const double* __restrict__ Dlo, int b, int k, i64 total, double* __restrict__ E) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const i64 S = e / ((i64)SB * SB);
+    const int r = (int)((e / SB) % SB), c = (int)(e % SB);
+    const int rb = r / b, cb = c / b;
+    E[e] = (rb == cb) ? Dlo[((i64)S * k + rb) * b * b + (i64)(r % b) * b + (c % b)] : 0.0;
+}
+
+// level d, first half: Tb(S, j) = sum_{m=j}^{j+d-1} L(Sk+j+d, Sk+m) E_S(m, j)
+template <typename T>
+__global__ __launch_bounds__(256) void super_level_t_kernel(const T* __restrict__ L, const double* __restrict__ E,
+                                                            double* __restrict__ Tb, int b, int n, int k, int d) {
+    __shared__ double As[16][65], Bs[16][65];
+    const int tpe = b / 64, tiles = tpe * tpe;
+    const int batch = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int S = batch / (k - d), j = batch % (k - d), i = j + d;
+    const int tm = tile / tpe, tn = tile % tpe;
+    double acc[4][4] = {};
+    const double* ES = E + (i64)S * SB * SB;
+    for (int m = j; m < i; ++m) {
+        const T* A = L + hs_block_id(S * k + i, S * k + m, n) * (i64)b * b + (i64)tm * 64 * b;
+        const double* B = ES + (i64)(m * b) * SB + j * b + tn * 64;
+        gemm64_acc(A, b, B, SB, b, acc, As, Bs);
+    }
+    double* out = Tb + ((i64)S * k + j) * b * b + (i64)tm * 64 * b + tn * 64;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[(i64)(4 * ty + r) * b + 4 * tx + c] = acc[r][c];
+}
+
+// level d, second half: E_S(j+d, j) = -L(Sk+j+d)^{-1} Tb(S, j)
+__global__ __launch_bounds__(256) void super_level_e_kernel(const double* __restrict__ Dlo, const double* __restrict__ Tb,
+                                                            double* __restrict__ E, int b, int k, int d) {
+    __shared__ double As[16][65], Bs[16][65];
+    const int tpe = b / 64, tiles = tpe * tpe;
+    const int batch = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int S = batch / (k - d), j = batch % (k - d), i = j + d;
+    const int tm = tile / tpe, tn = tile % tpe;
+    double acc[4][4] = {};
+    const double* A = Dlo + ((i64)S * k + i) * b * b + (i64)tm * 64 * b;
+    const double* B = Tb + ((i64)S * k + j) * b * b + tn * 64;
+    gemm64_acc(A, b, B, b, b, acc, As, Bs);
+    double* out = E + (i64)S * SB * SB + (i64)(i * b + tm * 64) * SB + j * b + tn * 64;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[(i64)(4 * ty + r) * SB + 4 * tx + c] = -acc[r][c];
+}
+
+// Et_S = E_S^T through 32 x 32 LDS tiles
+__global__ __launch_bounds__(256) void super_transpose_kernel(const double* __restrict__ E, double* __restrict__ Et) {
+    __shared__ double tl[32][33];
+    constexpr int TPE = SB / 32;
+    const i64 S = blockIdx.x / (TPE * TPE);
+    const int tile = blockIdx.x % (TPE * TPE);
+    const int tr = tile / TPE, tc = tile % TPE;
+    const double* src = E + S * SB * SB;
+    double* dst = Et + S * SB * SB;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) tl[r][tx] = src[(i64)(tr * 32 + r) * SB + tc * 32 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) dst[(i64)(tc * 32 + r) * SB + tr * 32 + tx] = tl[tx][r];
+}
+
+// Forward super-step update: y_I -= sum_{c<K} L(I, j0+c) z_{j0+c} for the
+// block rows I = I0 + g; the K blocks' z values sit in LDS, one row sum per
+// row over all K blocks (one reduction).
+template <typename T, int BS, int K>
+__global__ __launch_bounds__(TS_THREADS) void gemv_rows_multi_kernel(const T* __restrict__ L,
+                                                                     const double* __restrict__ zs,
+                                                                     double* __restrict__ y, int n, int I0, int j0,
+                                                                     int strips) {
+    constexpr int NL = BS / 128 > 0 ? BS / 128 : 1;
+    constexpr int LPR = BS / 2 < 64 ? BS / 2 : 64;
+    constexpr int RPL = 64 / LPR;
+    constexpr int ROWS = 16;
+    constexpr int R = ROWS / RPL;
+    __shared__ double vs[K * BS];
+    const int g = blockIdx.x / strips;
+    const int I = I0 + g;
+    const int r0 = (blockIdx.x % strips) * 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < K * BS; i += TS_THREADS) vs[i] = zs[i];
+    __syncthreads();
+    const int sub = lane / LPR, sl = lane % LPR;
+    double p[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) p[q] = 0.0;
+    for (int c = 0; c < K; ++c) {
+        const T* M = L + hs_block_id(I, j0 + c, n) * (i64)BS * BS;
+        const double* v = vs + c * BS;
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int r = r0 + wave * ROWS + kk * RPL + sub;
+            const T* row = M + (i64)r * BS + 2 * sl;
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < NL; ++q)
+                acc += ld(row + q * 128) * v[q * 128 + 2 * sl] + ld(row + q * 128 + 1) * v[q * 128 + 2 * sl + 1];
+            p[kk] += acc;
+        }
+    }
+    const double s = halving_reduce16<R, LPR>(p);
+    constexpr int SH = __builtin_ctz(LPR / R);
+    if ((sl & (LPR / R - 1)) == 0) {
+        const int r = r0 + wave * ROWS + (sl >> SH) * RPL + sub;
+        y[(i64)I * BS + r] -= s;
+    }
+}
+
+// Backward super-step update: z_I -= sum_{c<K} L(j0+c, I)^T x_{j0+c} for
+// I < j0, work-group (I, 64-column slice), lane = column, wave = a quarter of
+// each block's rows; the four wave sums are added in wave order.
+template <typename T, int BS, int K>
+__global__ __launch_bounds__(TS_THREADS) void gemv_cols_multi_kernel(const T* __restrict__ L,
+                                                                     const double* __restrict__ xs_g,
+                                                                     double* __restrict__ z, int n, int j0) {
+    constexpr int CW = 64;
+    constexpr int SL = BS / CW;
+    constexpr int RQ = BS / 4;
+    __shared__ double xs[K * BS];
+    __shared__ double red[4][CW];
+    const int I = blockIdx.x / SL;
+    const int c0 = (blockIdx.x % SL) * CW;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < K * BS; i += TS_THREADS) xs[i] = xs_g[i];
+    __syncthreads();
+    double acc = 0.0;
+    for (int c = 0; c < K; ++c) {
+        const T* col = L + hs_block_id(j0 + c, I, n) * (i64)BS * BS + c0 + lane;
+        const double* x = xs + c * BS;
+#pragma unroll 8
+        for (int kk = 0; kk < RQ; ++kk) {
+            const int r = wave * RQ + kk;
+            acc += ld(col + (i64)r * BS) * x[r];
+        }
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (t < CW) z[(i64)I * BS + c0 + t] -= (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+}
+
+inline int super_k(int b) { return SB / b; }
+inline int super_count(int n, int b) {
+    const int k = super_k(b);
+    return (k > 1 && n >= 2 * k) ? n / k : 0;
+}
+
 }  // namespace
 
 bool hs_trsv_fast_ok(int b) { return b == 64 || b == 128 || b == 256; }
 
-// Dinv holds two arrays of n b x b blocks: D_j = L_jj^{-T} then L_jj^{-1}.
+i64 hs_dinv_doubles(int n, int b) {
+    const i64 base = 2 * (i64)n * b * b;
+    if (!hs_trsv_fast_ok(b)) return base;
+    return base + 2 * (i64)super_count(n, b) * SB * SB;
+}
+
+// Dinv (hs_dinv_doubles(n, b) doubles) holds two arrays of n b x b blocks,
+// D_j = L_jj^{-T} then L_jj^{-1}, followed for the fast block sizes by the
+// super-block inverses E_S (SB x SB, row-major) and their transposes.
 template <typename T>
 int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv) {
     const int n = (int)((N + b - 1) / b);
     const int strips = b / 16;
+    const double* Dlo = Dinv + (i64)n * b * b;
     diag_inverse_kernel<T><<<dim3((unsigned)(n * strips)), dim3(TS_THREADS), 0, h->stream>>>(
         L, b, n, N, Dinv, Dinv + (i64)n * b * b, strips);
+    HS_LAUNCH_CHECK();
+    const int ns = hs_trsv_fast_ok(b) ? super_count(n, b) : 0;
+    if (ns == 0) return HS_OK;
+    const int k = super_k(b);
+    double* E = Dinv + 2 * (i64)n * b * b;
+    double* Et = E + (i64)ns * SB * SB;  // level scratch until the transpose
+    const i64 total = (i64)ns * SB * SB;
+    super_init_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream>>>(Dlo, b, k, total, E);
+    HS_LAUNCH_CHECK();
+    const int tiles = (b / 64) * (b / 64);
+    for (int d = 1; d < k; ++d) {
+        const unsigned grid = (unsigned)((i64)ns * (k - d) * tiles);
+        super_level_t_kernel<T><<<dim3(grid), dim3(256), 0, h->stream>>>(L, E, Et, b, n, k, d);
+        HS_LAUNCH_CHECK();
+        super_level_e_kernel<<<dim3(grid), dim3(256), 0, h->stream>>>(Dlo, Et, E, b, k, d);
+        HS_LAUNCH_CHECK();
+    }
+    super_transpose_kernel<<<dim3((unsigned)((i64)ns * (SB / 32) * (SB / 32))), dim3(256), 0, h->stream>>>(E, Et);
     HS_LAUNCH_CHECK();
     return HS_OK;
 }
@@ -251,11 +477,28 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
     const int strips = BS / 64;
     const double* Dup = Dinv;                      // D_j = L_jj^{-T}
     const double* Dlo = Dinv + (i64)n * BS * BS;   // L_jj^{-1}
+    constexpr int K = SB / BS;                     // blocks per super-block
+    const int ns = super_count(n, BS);
+    const double* E = Dinv + 2 * (i64)n * BS * BS;  // L_SS^{-1}
+    const double* Et = E + (i64)ns * SB * SB;       // L_SS^{-T}
+    const int jr = ns * K;                          // first block row past the super-blocks
     const dim3 blk(TS_THREADS);
     if (phases & 1) {
     int rc = hs_memset_zero(h, z, sizeof(double) * (size_t)n * BS);
     if (rc) return rc;
-    for (int j = 0; j < n; ++j) {
+    for (int S = 0; S < ns; ++S) {
+        // z_S = L_SS^{-1} y_S, then y_I -= sum_c L_I,c z_c below the super-block
+        gemv_rows_kernel<double, SB, 0, 2><<<dim3(SB / 8), blk, 0, h->stream>>>(
+            E + (i64)S * SB * SB, v + (i64)S * SB, z + (i64)S * SB, SB / 8, N - (i64)S * SB);
+        HS_LAUNCH_CHECK();
+        const int I0 = (S + 1) * K;
+        if (I0 < n) {
+            gemv_rows_multi_kernel<T, BS, K><<<dim3((unsigned)((n - I0) * strips)), blk, 0, h->stream>>>(
+                L, z + (i64)S * SB, v, n, I0, S * K, strips);
+            HS_LAUNCH_CHECK();
+        }
+    }
+    for (int j = jr; j < n; ++j) {
         // z_j = L_jj^{-1} y_j (entries past N stay zero)
         gemv_rows_kernel<double, BS, 0><<<dim3((unsigned)strips), blk, 0, h->stream>>>(
             Dlo + (i64)j * BS * BS, v + (i64)j * BS, z + (i64)j * BS, strips, N - (i64)j * BS);
@@ -269,7 +512,7 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
     }
     }
     if (phases & 2) {
-    for (int j = n - 1; j >= 0; --j) {
+    for (int j = n - 1; j >= jr; --j) {
         // x_j = L_jj^{-T} z_j (entries past N keep their zero)
         gemv_rows_kernel<double, BS, 0><<<dim3((unsigned)strips), blk, 0, h->stream>>>(
             Dup + (i64)j * BS * BS, z + (i64)j * BS, v + (i64)j * BS, strips, N - (i64)j * BS);
@@ -281,6 +524,17 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
                 gemv_cols_slice_kernel<T, BS><<<dim3((unsigned)(j * (BS / 64))), blk, 0, h->stream>>>(L, v, z, n, j);
             else
                 gemv_cols_update_kernel<T, BS><<<dim3((unsigned)j), blk, 0, h->stream>>>(L, v, z, n, j);
+            HS_LAUNCH_CHECK();
+        }
+    }
+    for (int S = ns - 1; S >= 0; --S) {
+        // x_S = L_SS^{-T} z_S, then z_I -= sum_c L_c,I^T x_c above the super-block
+        gemv_rows_kernel<double, SB, 0, 2><<<dim3(SB / 8), blk, 0, h->stream>>>(
+            Et + (i64)S * SB * SB, z + (i64)S * SB, v + (i64)S * SB, SB / 8, N - (i64)S * SB);
+        HS_LAUNCH_CHECK();
+        if (S > 0) {
+            gemv_cols_multi_kernel<T, BS, K><<<dim3((unsigned)(S * K * (BS / 64))), blk, 0, h->stream>>>(
+                L, v + (i64)S * SB, z, n, S * K);
             HS_LAUNCH_CHECK();
         }
     }

@@ -85,7 +85,9 @@ def test_potrf_and_trsm_match_oracle(hs, dev, oracle, b):
 
 @pytest.mark.parametrize("N,b", [(1024, 64), (1000, 64), (2048, 128), (1536, 256), (500, 20),
                                  # two-panel look-ahead edge cases: 4, 5 and 7 block columns, padded or not
-                                 (512, 128), (1280, 256), (1100, 256), (448, 64), (330, 64)])
+                                 (512, 128), (1280, 256), (1100, 256), (448, 64), (330, 64),
+                                 # 1024-row super-block steps: padded last super-block, remainder blocks
+                                 (2000, 128), (2300, 256), (3000, 64)])
 def test_cholesky_and_solve_match_oracle(hs, dev, oracle, N, b):
     say("cholesky+solve N=%d b=%d" % (N, b))
     X, A, rhs = rbf_system(oracle, N, 8, b)
