@@ -340,10 +340,19 @@ __global__ __launch_bounds__(256) void super_transpose_kernel(const double* __re
     for (int r = ty; r < 32; r += 8) dst[(i64)(tc * 32 + r) * SB + tr * 32 + tx] = tl[tx][r];
 }
 
+// Two adjacent stored values, loaded as one 16-byte (double) or 8-byte
+// (float) access.
+template <typename T>
+using pair_t = T __attribute__((ext_vector_type(2)));
+
 // Forward super-step update: y_I -= sum_{c<K} L(I, j0+c) z_{j0+c} for the
-// block rows I = I0 + g; the K blocks' z values sit in LDS, one row sum per
-// row over all K blocks (one reduction).
-template <typename T, int BS, int K>
+// block rows I = I0 + g. Work-group (g, strip) owns ROWS rows of block row I;
+// wave w takes the blocks c = w, w+4, ... and stages all of a block's loads
+// in registers before its products, so that they are in flight together
+// (written as one accumulation loop the compiler waits on each load in
+// turn). The four wave sums are added in wave order. The K blocks' z values
+// sit in LDS.
+template <typename T, int BS, int K, int ROWS>
 __global__ __launch_bounds__(TS_THREADS) void gemv_rows_multi_kernel(const T* __restrict__ L,
                                                                      const double* __restrict__ zs,
                                                                      double* __restrict__ y, int n, int I0, int j0,
@@ -351,12 +360,13 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_rows_multi_kernel(const T* __
     constexpr int NL = BS / 128 > 0 ? BS / 128 : 1;
     constexpr int LPR = BS / 2 < 64 ? BS / 2 : 64;
     constexpr int RPL = 64 / LPR;
-    constexpr int ROWS = 16;
     constexpr int R = ROWS / RPL;
+    static_assert(K % 4 == 0, "four waves share the blocks evenly");
     __shared__ double vs[K * BS];
+    __shared__ double red[4][ROWS];
     const int g = blockIdx.x / strips;
     const int I = I0 + g;
-    const int r0 = (blockIdx.x % strips) * 64;
+    const int r0 = (blockIdx.x % strips) * ROWS;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int i = t; i < K * BS; i += TS_THREADS) vs[i] = zs[i];
     __syncthreads();
@@ -364,58 +374,79 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_rows_multi_kernel(const T* __
     double p[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) p[q] = 0.0;
-    for (int c = 0; c < K; ++c) {
-        const T* M = L + hs_block_id(I, j0 + c, n) * (i64)BS * BS;
-        const double* v = vs + c * BS;
+    for (int c = wave; c < K; c += 4) {
+        const T* M = L + hs_block_id(I, j0 + c, n) * (i64)BS * BS + (i64)(r0 + sub) * BS + 2 * sl;
+        pair_t<T> raw[R][NL];
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk)
+#pragma unroll
+            for (int q = 0; q < NL; ++q) raw[kk][q] = *(const pair_t<T>*)(M + (i64)kk * RPL * BS + q * 128);
+        const double* v = vs + c * BS + 2 * sl;
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
-            const int r = r0 + wave * ROWS + kk * RPL + sub;
-            const T* row = M + (i64)r * BS + 2 * sl;
             double acc = 0.0;
 #pragma unroll
             for (int q = 0; q < NL; ++q)
-                acc += ld(row + q * 128) * v[q * 128 + 2 * sl] + ld(row + q * 128 + 1) * v[q * 128 + 2 * sl + 1];
+                acc += (double)raw[kk][q].x * v[q * 128] + (double)raw[kk][q].y * v[q * 128 + 1];
             p[kk] += acc;
         }
     }
     const double s = halving_reduce16<R, LPR>(p);
     constexpr int SH = __builtin_ctz(LPR / R);
-    if ((sl & (LPR / R - 1)) == 0) {
-        const int r = r0 + wave * ROWS + (sl >> SH) * RPL + sub;
-        y[(i64)I * BS + r] -= s;
-    }
+    if ((sl & (LPR / R - 1)) == 0) red[wave][(sl >> SH) * RPL + sub] = s;
+    __syncthreads();
+    if (t < ROWS) y[(i64)I * BS + r0 + t] -= (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
+// rows per work-group of the forward super-step update
+template <int BS>
+constexpr int multi_rows() { return BS == 64 ? 16 : 8; }
+
 // Backward super-step update: z_I -= sum_{c<K} L(j0+c, I)^T x_{j0+c} for
-// I < j0, work-group (I, 64-column slice), lane = column, wave = a quarter of
-// each block's rows; the four wave sums are added in wave order.
+// I < j0. Work-group (I, 64-column slice): lane l reads columns 2(l%32),
+// 2(l%32)+1 of every other row (l/32 picks the row parity) and wave w a
+// quarter of each block's rows, in staged chunks of up to 32 rows whose loads
+// are all issued before their products. The eight partial column sums
+// (4 waves x 2 parities) are added in a fixed order.
 template <typename T, int BS, int K>
 __global__ __launch_bounds__(TS_THREADS) void gemv_cols_multi_kernel(const T* __restrict__ L,
                                                                      const double* __restrict__ xs_g,
                                                                      double* __restrict__ z, int n, int j0) {
     constexpr int CW = 64;
     constexpr int SL = BS / CW;
-    constexpr int RQ = BS / 4;
+    constexpr int RQ = BS / 4;               // rows per wave and block
+    constexpr int CH = RQ < 32 ? RQ : 32;    // rows per staged chunk
     __shared__ double xs[K * BS];
-    __shared__ double red[4][CW];
+    __shared__ double red[8][CW];
     const int I = blockIdx.x / SL;
     const int c0 = (blockIdx.x % SL) * CW;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int par = lane >> 5, cl = lane & 31;
     for (int i = t; i < K * BS; i += TS_THREADS) xs[i] = xs_g[i];
     __syncthreads();
-    double acc = 0.0;
+    double acc0 = 0.0, acc1 = 0.0;
     for (int c = 0; c < K; ++c) {
-        const T* col = L + hs_block_id(j0 + c, I, n) * (i64)BS * BS + c0 + lane;
-        const double* x = xs + c * BS;
-#pragma unroll 8
-        for (int kk = 0; kk < RQ; ++kk) {
-            const int r = wave * RQ + kk;
-            acc += ld(col + (i64)r * BS) * x[r];
+        const T* base = L + hs_block_id(j0 + c, I, n) * (i64)BS * BS + c0 + 2 * cl;
+        const double* x = xs + c * BS + par;
+#pragma unroll
+        for (int r0 = wave * RQ; r0 < wave * RQ + RQ; r0 += CH) {
+            pair_t<T> raw[CH / 2];
+#pragma unroll
+            for (int k = 0; k < CH / 2; ++k) raw[k] = *(const pair_t<T>*)(base + (i64)(r0 + 2 * k + par) * BS);
+#pragma unroll
+            for (int k = 0; k < CH / 2; ++k) {
+                const double xr = x[r0 + 2 * k];
+                acc0 += (double)raw[k].x * xr;
+                acc1 += (double)raw[k].y * xr;
+            }
         }
     }
-    red[wave][lane] = acc;
+    red[2 * wave + par][2 * cl] = acc0;
+    red[2 * wave + par][2 * cl + 1] = acc1;
     __syncthreads();
-    if (t < CW) z[(i64)I * BS + c0 + t] -= (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    if (t < CW)
+        z[(i64)I * BS + c0 + t] -= ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t])) +
+                                   ((red[4][t] + red[5][t]) + (red[6][t] + red[7][t]));
 }
 
 inline int super_k(int b) { return SB / b; }
@@ -493,8 +524,9 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
         HS_LAUNCH_CHECK();
         const int I0 = (S + 1) * K;
         if (I0 < n) {
-            gemv_rows_multi_kernel<T, BS, K><<<dim3((unsigned)((n - I0) * strips)), blk, 0, h->stream>>>(
-                L, z + (i64)S * SB, v, n, I0, S * K, strips);
+            constexpr int MR = multi_rows<BS>();
+            gemv_rows_multi_kernel<T, BS, K, MR><<<dim3((unsigned)((n - I0) * (BS / MR))), blk, 0, h->stream>>>(
+                L, z + (i64)S * SB, v, n, I0, S * K, BS / MR);
             HS_LAUNCH_CHECK();
         }
     }
