@@ -405,24 +405,25 @@ constexpr int multi_rows() { return BS == 64 ? 16 : 8; }
 // Backward super-step update: z_I -= sum_{c<K} L(j0+c, I)^T x_{j0+c} for
 // I < j0. Work-group (I, 64-column slice): lane l reads columns 2(l%32),
 // 2(l%32)+1 of every other row (l/32 picks the row parity) and wave w a
-// quarter of each block's rows, in staged chunks of up to 32 rows whose loads
-// are all issued before their products. The eight partial column sums
-// (4 waves x 2 parities) are added in a fixed order.
-template <typename T, int BS, int K>
-__global__ __launch_bounds__(TS_THREADS) void gemv_cols_multi_kernel(const T* __restrict__ L,
-                                                                     const double* __restrict__ xs_g,
-                                                                     double* __restrict__ z, int n, int j0) {
+// 1/W share of each block's rows, in staged chunks of up to 32 rows whose
+// loads are all issued before their products. The 2W partial column sums are
+// added in a fixed order. W = 16 for the short steps near the top, where
+// few work-groups each stream K blocks.
+template <typename T, int BS, int K, int W>
+__global__ __launch_bounds__(64 * W) void gemv_cols_multi_kernel(const T* __restrict__ L,
+                                                                 const double* __restrict__ xs_g,
+                                                                 double* __restrict__ z, int n, int j0) {
     constexpr int CW = 64;
     constexpr int SL = BS / CW;
-    constexpr int RQ = BS / 4;               // rows per wave and block
+    constexpr int RQ = BS / W;               // rows per wave and block
     constexpr int CH = RQ < 32 ? RQ : 32;    // rows per staged chunk
     __shared__ double xs[K * BS];
-    __shared__ double red[8][CW];
+    __shared__ double red[2 * W][CW];
     const int I = blockIdx.x / SL;
     const int c0 = (blockIdx.x % SL) * CW;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int par = lane >> 5, cl = lane & 31;
-    for (int i = t; i < K * BS; i += TS_THREADS) xs[i] = xs_g[i];
+    for (int i = t; i < K * BS; i += 64 * W) xs[i] = xs_g[i];
     __syncthreads();
     double acc0 = 0.0, acc1 = 0.0;
     for (int c = 0; c < K; ++c) {
@@ -444,9 +445,12 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_cols_multi_kernel(const T* __
     red[2 * wave + par][2 * cl] = acc0;
     red[2 * wave + par][2 * cl + 1] = acc1;
     __syncthreads();
-    if (t < CW)
-        z[(i64)I * BS + c0 + t] -= ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t])) +
-                                   ((red[4][t] + red[5][t]) + (red[6][t] + red[7][t]));
+    if (t < CW) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2 * W; q += 4) s += (red[q][t] + red[q + 1][t]) + (red[q + 2][t] + red[q + 3][t]);
+        z[(i64)I * BS + c0 + t] -= s;
+    }
 }
 
 inline int super_k(int b) { return SB / b; }
@@ -565,8 +569,12 @@ static int tri_solve_bs(hs_handle_t h, const T* L, const double* Dinv, double* v
             Et + (i64)S * SB * SB, z + (i64)S * SB, v + (i64)S * SB, SB / 8, N - (i64)S * SB);
         HS_LAUNCH_CHECK();
         if (S > 0) {
-            gemv_cols_multi_kernel<T, BS, K><<<dim3((unsigned)(S * K * (BS / 64))), blk, 0, h->stream>>>(
-                L, v + (i64)S * SB, z, n, S * K);
+            const unsigned wgs = (unsigned)(S * K * (BS / 64));
+            if (wgs < 2u * (unsigned)h->cu_count)
+                gemv_cols_multi_kernel<T, BS, K, 16><<<dim3(wgs), dim3(1024), 0, h->stream>>>(
+                    L, v + (i64)S * SB, z, n, S * K);
+            else
+                gemv_cols_multi_kernel<T, BS, K, 4><<<dim3(wgs), blk, 0, h->stream>>>(L, v + (i64)S * SB, z, n, S * K);
             HS_LAUNCH_CHECK();
         }
     }

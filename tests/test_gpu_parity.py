@@ -127,6 +127,46 @@ def test_solve_within_1e12_when_well_conditioned(hs, dev, oracle, N, b, sn2):
     assert rel <= 1e-12
 
 
+@pytest.mark.parametrize("N,b", [(40960, 256), (36864, 128)])
+def test_solve_recovers_known_solution_at_scale(hs, dev, N, b):
+    """Sizes past the oracle's reach, where the backward super-steps near the
+    bottom use their wide-grid form: y = K x_true (SYMV on the GPU), factor,
+    solve, and x must match x_true to cond(K) * u with a small backward
+    error."""
+    say("known-solution solve N=%d b=%d" % (N, b))
+    d, ell, sf2, sn2 = 8, 1.0, 1.0, 1e-1
+    n = -(-N // b)
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1.0, 1.0, (N, d))
+    xt = np.zeros(n * b)
+    xt[:N] = rng.standard_normal(N)
+    dX, dxt, dy = dev.upload(X.reshape(-1)), dev.upload(xt), dev.zeros(n * b)
+    dA = dev.alloc(packed_len(N, b) * 8)
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dxt.p, dy.p, N, b)
+    y = dev.download(dy, n * b)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dy.p, N, b)
+    x = dev.download(dy, n * b)
+    # residual against a fresh K
+    hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dA.p)
+    dx, dr = dev.upload(x), dev.zeros(n * b)
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dx.p, dr.p, N, b)
+    r = y - dev.download(dr, n * b)
+    ones = np.zeros(n * b)
+    ones[:N] = 1.0
+    hs.call("hs_symv_packed_f64", dev.h, dA.p, dev.upload(ones).p, dr.p, N, b)
+    knorm = float(np.max(dev.download(dr, n * b)))
+    dA.free()
+    bwd = np.max(np.abs(r)) / (knorm * np.max(np.abs(x)) + np.max(np.abs(y)))
+    rel = np.linalg.norm(x - xt) / np.linalg.norm(xt)
+    say("  backward error %.3e  rel error vs x_true %.3e" % (bwd, rel))
+    # cond(K) <= 1 + N sf2 / sn2 < 5e5 here
+    assert bwd <= 1e-15
+    assert rel <= 1e-8
+    assert np.all(x[N:] == 0.0)
+
+
 def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
     N, b = 256, 64
     K = np.eye(N)
