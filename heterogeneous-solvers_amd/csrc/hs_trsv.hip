@@ -240,21 +240,31 @@ constexpr int SB = 1024;
 
 // 64 x 64 tile of C = sum_t A_t B_t (A row-major of any stored type, B row
 // major double), 256 threads with a 4 x 4 register tile each, 16-deep k
-// stages through LDS.
+// stages through LDS; the next stage's loads are in flight (in registers)
+// while the current stage is multiplied.
 template <typename TA>
 __device__ inline void gemm64_acc(const TA* A, i64 lda, const double* B, i64 ldb, int K, double (&acc)[4][4],
                                   double (*As)[65], double (*Bs)[65]) {
     const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = t + 256 * e;  // 0 .. 1023
+            ra[e] = ld(A + (i64)(idx >> 4) * lda + k0 + (idx & 15));
+            rb[e] = B[(i64)(k0 + (idx >> 6)) * ldb + (idx & 63)];
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int idx = t + 256 * e;         // 0 .. 1023
-            const int ar = idx >> 4, ak = idx & 15;
-            As[ak][ar] = ld(A + (i64)ar * lda + k0 + ak);
-            const int bk = idx >> 6, bc = idx & 63;
-            Bs[bk][bc] = B[(i64)(k0 + bk) * ldb + bc];
+            const int idx = t + 256 * e;
+            As[idx & 15][idx >> 4] = ra[e];
+            Bs[idx >> 6][idx & 63] = rb[e];
         }
         __syncthreads();
+        if (k0 + 16 < K) fetch(k0 + 16);
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
             double a[4], bv[4];
