@@ -65,6 +65,12 @@ def rest_update_work(hs, N, b):
     return cnt.value, fl.value
 
 
+def mixed_update_work(hs, N, b, variant):
+    cnt, fl = C.c_int64(), C.c_double()
+    hs.call("hs_cholesky_mixed_schedule", N, b, variant, C.byref(cnt), C.byref(fl))
+    return cnt.value, fl.value
+
+
 def cholesky_workload(hs, dev, syn, args):
     seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
     if args.N:
@@ -185,7 +191,7 @@ def mixed_workload(hs, dev, syn, args):
         fac = float(np.median([r[0] for r in runs]))
         ref = float(np.median([r[1] for r in runs]))
         ph = np.mean([r[2] for r in runs], axis=0)
-        _, upd_flops = rest_update_work(hs, N, b)
+        _, upd_flops = mixed_update_work(hs, N, b, variant)
         out[name] = {"factor_ms": fac, "refine_ms": ref, "solve_ms": fac + ref, "report": runs[-1][3],
                      "update_kernel_ms": float(ph[2]), "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
                      "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}

@@ -265,8 +265,8 @@ inline int hs_schedule_panels(int n) {
 // updates the m = n - k - 2P block rows from row k + 2P with P panels, i.e.
 // P b^3 m^2 flops (2 P b^3 per off-diagonal block, P b^3 per diagonal block,
 // whose upper half is not needed).
-inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops) {
-    const int P = hs_schedule_panels(n);
+inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops, int panels = 0) {
+    const int P = panels > 0 ? std::max(1, std::min(panels, n / 2)) : hs_schedule_panels(n);
     int64_t cnt = 0;
     double f = 0.0;
     for (int k = 0; k + P < n; k += P) {
@@ -292,9 +292,11 @@ inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops) 
 // those columns.
 //   potrf(k), trsm(k, blockStart, blockCount): launch on h->stream
 //   trail(stream, k, mode, rowStart, rowCount, np): launch on `stream`
+// panels = 0 takes hs_schedule_panels(n); any other value is clamped to
+// 1 .. n/2.
 template <class Potrf, class Trsm, class Trail>
-int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm trsm, Trail trail) {
-    const int P = hs_schedule_panels(n);
+int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm trsm, Trail trail, int panels = 0) {
+    const int P = panels > 0 ? std::max(1, std::min(panels, n / 2)) : hs_schedule_panels(n);
     int rc = hs_ensure_events(h, 2 * (size_t)n + 2);
     if (rc) return rc;
     hipEvent_t* evP = h->events.data();          // panels of step s final (S2)

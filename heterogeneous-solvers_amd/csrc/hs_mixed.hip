@@ -545,15 +545,34 @@ extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int6
     return HS_OK;
 }
 
+// Panels per step of the mixed factor's look-ahead schedule: 6. Every rest-
+// update launch reads and writes its FP32 C tiles once, so deeper updates
+// spend less of the launch in that epilogue (BF16 hi+lo 256-tile update
+// alone: 1165 / 1224 / 1257 TFLOP/s over the three products for 4 / 6 / 8
+// panels), against a longer panel chain per step. Factor at N = 65536,
+// b = 256 for 4 / 6 / 8 panels: BF16 hi+lo 324 / 318 / 322 ms, FP32 808 / 797
+// / 806 ms. HS_MIXED_PANELS = 1..8 chooses another depth.
+static int mixed_panels(int n, int variant) {
+    (void)variant;
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("HS_MIXED_PANELS");
+        const int w = e ? std::atoi(e) : 0;
+        v = (w >= 1 && w <= 8) ? w : 6;
+    }
+    return std::max(1, std::min(v, n / 2));
+}
+
 static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, PhaseTimer& pt) {
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    // bf16 hi / lo planes of one rest update's panels: at most 4 panels of
-    // n block rows, 4 bytes per element in all
+    const int P = mixed_panels(n, variant);
+    // bf16 hi / lo planes of one rest update's panels: P panels of at most n
+    // block rows, 4 bytes per element in all
     unsigned short* planes = nullptr;
     if (variant == HS_MIXED_BF16X3) {
-        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)4 * n * b * b))) return rc;
+        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)P * n * b * b))) return rc;
         planes = (unsigned short*)h->work;
     }
     auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
@@ -563,13 +582,26 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
         return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np, planes);
     };
-    if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail))) {
+    if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail, P))) {
         hipStreamSynchronize(h->stream2);
         return rc;
     }
     int64_t info = 0;
     if ((rc = hs_get_info(h, &info))) return rc;
     return (int)info;
+}
+
+// Work of the rest-update launches of the mixed factor's schedule for N, b
+// and variant: their count and algorithmic flops (one product per element).
+extern "C" int hs_cholesky_mixed_schedule(int64_t N, int b, int variant, int64_t* rest_launches, double* rest_flops) {
+    if (N <= 0) return -1;
+    if (b % 128 != 0 || b > 256) return -2;
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -3;
+    if (!rest_launches) return -4;
+    if (!rest_flops) return -5;
+    const int n = (int)((N + b - 1) / b);
+    hs_rest_update_work(n, b, rest_launches, rest_flops, mixed_panels(n, variant));
+    return HS_OK;
 }
 
 extern "C" int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant,

@@ -156,11 +156,16 @@ int hs_cholesky_f64(hs_handle_t h, double* A, int64_t N, int b);
  * reference collects into MetricsTracker (Cholesky.cpp:261-359). */
 int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* phase_ms, int64_t* launches);
 
-/* The rest-update launches of the look-ahead schedule hs_cholesky_f64 and
- * hs_cholesky_mixed use for N and b (b a multiple of 64): their number and
+/* The rest-update launches of the look-ahead schedule hs_cholesky_f64 uses
+ * for N and b (b a multiple of 64): their number and
  * their algorithmic flops (b^3 per diagonal block and 2 b^3 per off-diagonal
  * block and panel). Measurement helper; no reference counterpart. */
 int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops);
+
+/* The same for the look-ahead schedule of hs_cholesky_mixed with this variant
+ * (b = 128 or 256), whose steps may apply more panels than the FP64 one.
+ * Measurement helper; no reference counterpart. */
+int hs_cholesky_mixed_schedule(int64_t N, int b, int variant, int64_t* rest_launches, double* rest_flops);
 
 /* Forward L z = x then backward L^T y = z, in place on x (device).
  * Replaces TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105). */
