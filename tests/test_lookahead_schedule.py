@@ -1,0 +1,133 @@
+"""CPU model of the two-stream look-ahead schedule of the blocked Cholesky
+(hs_lookahead_factor in heterogeneous-solvers_amd/csrc/hs_factor.h): the
+launch sequence of both streams with their event waits, replayed with vector
+clocks. Every pair of launches on different streams that is not ordered by
+the events must touch disjoint blocks (no write shared with a read or a
+write), and executing the launches in an order the events allow must give
+numpy's Cholesky. Covers the panel counts the FP64 (4) and mixed (6) factors
+use and the clamping to n / 2."""
+import itertools
+
+import numpy as np
+import pytest
+
+def pairs_for(mode, k, n, np_, extra=None):
+    if mode == 4:
+        rs = k + 2 * np_
+        return [(I, J) for J in range(rs, n) for I in range(J, n)]
+    if mode == 5:
+        rs = k + 1
+        return [(I, rs) for I in range(rs, n)]
+    if mode == 6:
+        rs, nc = extra
+        out = []
+        for c in range(min(nc, n - rs)):
+            out += [(I, rs + c) for I in range(rs + c, n)]
+        return out
+    raise ValueError
+
+def schedule(n, P):
+    ops = []  # (stream, kind, args) or ('rec', stream, ev) / ('wait', stream, ev)
+    S1, S2 = 1, 2
+    ops.append(('rec', S1, ('U', n))); ops.append(('wait', S2, ('U', n)))
+    def chain(k0):
+        kend = min(n, k0 + P)
+        for k in range(k0, kend):
+            ops.append((S2, 'potrf', (k,)))
+            if k + 1 < n: ops.append((S2, 'trsm', (k, k + 1, n - k - 1)))
+            if k + 1 < kend:
+                if k + 2 == kend: ops.append((S2, 'trail', (k, 5, 1)))
+                else: ops.append((S2, 'trail', (k, 6, 1, (k + 1, kend - k - 1))))
+    chain(0); ops.append(('rec', S2, ('P', 0)))
+    step = 0; k = 0
+    while k + P < n:
+        if step >= 1: ops.append(('wait', S2, ('U', step - 1)))
+        ops.append((S2, 'trail', (k, 5, 1) if P == 1 else (k, 6, P, (k + P, P))))
+        chain(k + P); ops.append(('rec', S2, ('P', step + 1)))
+        ops.append(('wait', S1, ('P', step)))
+        if k + 2 * P < n: ops.append((S1, 'trail', (k, 4, P)))
+        ops.append(('rec', S1, ('U', step)))
+        k += P; step += 1
+    ops.append(('wait', S1, ('P', step)))
+    return ops
+
+def rw_sets(op, n):
+    kind, a = op[1], op[2]
+    if kind == 'potrf': return {(a[0], a[0])}, {(a[0], a[0])}
+    if kind == 'trsm':
+        k, st, c = a
+        return {(k, k)} | {(I, k) for I in range(st, st + c)}, {(I, k) for I in range(st, st + c)}
+    k, mode, np_ = a[0], a[1], a[2]
+    ps = pairs_for(mode, k, n, np_, a[3] if len(a) > 3 else None)
+    R = set(ps); W = set(ps)
+    for (I, J) in ps:
+        for p in range(np_): R |= {(I, k + p), (J, k + p)}
+    return R, W
+
+def check(n, P, b=3):
+    ops = schedule(n, P)
+    # vector clocks: per stream op index; event -> clock snapshot
+    clocks = {1: {1: 0, 2: 0}, 2: {1: 0, 2: 0}}
+    ev = {}
+    real = []  # (stream, idx, clock_at_start, op)
+    cnt = {1: 0, 2: 0}
+    for op in ops:
+        if op[0] == 'rec':
+            ev[op[2]] = dict(clocks[op[1]])
+        elif op[0] == 'wait':
+            e = ev[op[2]]
+            for s_ in (1, 2): clocks[op[1]][s_] = max(clocks[op[1]][s_], e[s_])
+        else:
+            s_ = op[0]; cnt[s_] += 1
+            real.append((s_, cnt[s_], dict(clocks[s_]), op))
+            clocks[s_][s_] = cnt[s_]
+    # a happens-before b iff b's start clock for a's stream >= a's index
+    def hb(a, b): return b[2][a[0]] >= a[1]
+    races = 0
+    for a, c in itertools.combinations(real, 2):
+        if a[0] == c[0]: continue
+        if hb(a, c) or hb(c, a): continue
+        Ra, Wa = rw_sets(a[3], n); Rc, Wc = rw_sets(c[3], n)
+        if (Wa & (Rc | Wc)) or (Wc & Ra):
+            races += 1
+    # numeric: execute in program order of a topological merge (S2 first when possible)
+    rng = np.random.default_rng(n * 10 + P)
+    N = n * b
+    M = rng.standard_normal((N, N)); K = M @ M.T + N * np.eye(N)
+    A = np.tril(K.copy())
+    blk = lambda I, J: (slice(I * b, I * b + b), slice(J * b, J * b + b))
+    q = {1: [r for r in real if r[0] == 1], 2: [r for r in real if r[0] == 2]}
+    pos = {1: 0, 2: 0}; order = []
+    def ready(c):
+        other = 1 if c[0] == 2 else 2
+        return pos[other] >= c[2][other]   # all ops of the other stream it waits on are done
+    while pos[1] < len(q[1]) or pos[2] < len(q[2]):
+        progressed = False
+        for s_ in (2, 1):
+            if pos[s_] < len(q[s_]) and ready(q[s_][pos[s_]]):
+                order.append(q[s_][pos[s_]]); pos[s_] += 1; progressed = True; break
+        assert progressed, "deadlock"
+    for s_, i, _, op in order:
+        kind, a = op[1], op[2]
+        if kind == 'potrf':
+            k = a[0]; A[blk(k, k)] = np.linalg.cholesky(np.tril(A[blk(k, k)]) + np.tril(A[blk(k, k)], -1).T)
+        elif kind == 'trsm':
+            k, st, c_ = a; L = A[blk(k, k)]
+            for I in range(st, st + c_): A[blk(I, k)] = np.linalg.solve(L, A[blk(I, k)].T).T
+        else:
+            k, mode, np_ = a[0], a[1], a[2]
+            for (I, J) in pairs_for(mode, k, n, np_, a[3] if len(a) > 3 else None):
+                for p in range(np_):
+                    A[blk(I, J)] -= A[blk(I, k + p)] @ A[blk(J, k + p)].T
+                if I == J: A[blk(I, J)] = np.tril(A[blk(I, J)])
+    err = np.abs(np.tril(A) - np.linalg.cholesky(K)).max()
+    return races, err
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_lookahead_schedule_is_race_free_and_exact(P):
+    for n in range(1, 20):
+        p = max(1, min(P, n // 2))
+        races, err = check(n, p)
+        assert races == 0, (n, p)
+        assert err < 1e-8, (n, p, err)
