@@ -377,8 +377,21 @@ def cpu_baseline(args):
     o.tri_solve(L, rhs, N, b)
     dt = time.time() - t0
     return {"value": N ** 3 / 3.0 / dt / 1e12, "unit": "TFLOP/s", "cores": int(o.or_num_threads()), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "oracle (C/OpenMP restatement of the reference CPU path): assembly + blocked Cholesky + "
                       "2 triangular solves at N=%d, b=%d, d=%d, %.1f s" % (N, b, d, dt)}
+
+
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (None where it is not readable)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
