@@ -238,14 +238,19 @@ __global__ __launch_bounds__(TS_THREADS) void gemv_cols_slice_kernel(const T* __
 // columns: 2n / k dependent launches per solve instead of 2n.
 constexpr int SB = 1024;
 
-// 64 x 64 tile of C = sum_t A_t B_t (A row-major of any stored type, B row
-// major double), 256 threads with a 4 x 4 register tile each, 16-deep k
-// stages through LDS; the next stage's loads are in flight (in registers)
-// while the current stage is multiplied.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// 64 x 64 tile of C += A B (A row-major of any stored type, B row-major
+// double) on v_mfma_f64_16x16x4f64: 256 threads as 2 x 2 waves of 32 x 32,
+// 16-deep k stages through LDS (A stored k-major), the next stage's loads in
+// registers while the current one is multiplied. acc[i][j] register v of
+// lane l is row 32 wm + 16 i + l/16 + 4v, column 32 wn + 16 j + l%16.
 template <typename TA>
-__device__ inline void gemm64_acc(const TA* A, i64 lda, const double* B, i64 ldb, int K, double (&acc)[4][4],
-                                  double (*As)[65], double (*Bs)[65]) {
-    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+__device__ inline void gemm64_mfma(const TA* A, i64 lda, const double* B, i64 ldb, int K, d4 (&acc)[2][2],
+                                   double (*As)[68], double (*Bs)[68]) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int m0 = (wave >> 1) * 32, n0 = (wave & 1) * 32;
+    const int lr = lane & 15, lq = lane >> 4;
     double ra[4], rb[4];
     auto fetch = [&](int k0) {
 #pragma unroll
@@ -266,19 +271,33 @@ __device__ inline void gemm64_acc(const TA* A, i64 lda, const double* B, i64 ldb
         __syncthreads();
         if (k0 + 16 < K) fetch(k0 + 16);
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double a[4], bv[4];
+        for (int ks = 0; ks < 16; ks += 4) {
+            double av[2], bv[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][4 * ty + i];
+            for (int i = 0; i < 2; ++i) av[i] = As[ks + lq][m0 + 16 * i + lr];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][4 * tx + j];
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[ks + lq][n0 + 16 * j + lr];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bv[j];
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
+}
+
+// out[r][c] = sg * acc for the 64 x 64 tile (row pitch ldo)
+__device__ inline void store64(double* out, i64 ldo, const d4 (&acc)[2][2], double sg) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = (wave >> 1) * 32, n0 = (wave & 1) * 32;
+    const int lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[(i64)(m0 + 16 * i + lq + 4 * v) * ldo + n0 + 16 * j + lr] = sg * acc[i][j][v];
 }
 
 // E_S <- the block inverses on its diagonal, zero elsewhere
@@ -295,44 +314,42 @@ __global__ void super_init_kernel(const double* __restrict__ Dlo, int b, int k, 
 template <typename T>
 __global__ __launch_bounds__(256) void super_level_t_kernel(const T* __restrict__ L, const double* __restrict__ E,
                                                             double* __restrict__ Tb, int b, int n, int k, int d) {
-    __shared__ double As[16][65], Bs[16][65];
+    __shared__ double As[16][68], Bs[16][68];
     const int tpe = b / 64, tiles = tpe * tpe;
     const int batch = blockIdx.x / tiles, tile = blockIdx.x % tiles;
     const int S = batch / (k - d), j = batch % (k - d), i = j + d;
     const int tm = tile / tpe, tn = tile % tpe;
-    double acc[4][4] = {};
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[a][c] = d4{0.0, 0.0, 0.0, 0.0};
     const double* ES = E + (i64)S * SB * SB;
     for (int m = j; m < i; ++m) {
         const T* A = L + hs_block_id(S * k + i, S * k + m, n) * (i64)b * b + (i64)tm * 64 * b;
         const double* B = ES + (i64)(m * b) * SB + j * b + tn * 64;
-        gemm64_acc(A, b, B, SB, b, acc, As, Bs);
+        gemm64_mfma(A, b, B, SB, b, acc, As, Bs);
     }
-    double* out = Tb + ((i64)S * k + j) * b * b + (i64)tm * 64 * b + tn * 64;
-    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[(i64)(4 * ty + r) * b + 4 * tx + c] = acc[r][c];
+    store64(Tb + ((i64)S * k + j) * b * b + (i64)tm * 64 * b + tn * 64, b, acc, 1.0);
 }
 
 // level d, second half: E_S(j+d, j) = -L(Sk+j+d)^{-1} Tb(S, j)
 __global__ __launch_bounds__(256) void super_level_e_kernel(const double* __restrict__ Dlo, const double* __restrict__ Tb,
                                                             double* __restrict__ E, int b, int k, int d) {
-    __shared__ double As[16][65], Bs[16][65];
+    __shared__ double As[16][68], Bs[16][68];
     const int tpe = b / 64, tiles = tpe * tpe;
     const int batch = blockIdx.x / tiles, tile = blockIdx.x % tiles;
     const int S = batch / (k - d), j = batch % (k - d), i = j + d;
     const int tm = tile / tpe, tn = tile % tpe;
-    double acc[4][4] = {};
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[a][c] = d4{0.0, 0.0, 0.0, 0.0};
     const double* A = Dlo + ((i64)S * k + i) * b * b + (i64)tm * 64 * b;
     const double* B = Tb + ((i64)S * k + j) * b * b + tn * 64;
-    gemm64_acc(A, b, B, b, b, acc, As, Bs);
-    double* out = E + (i64)S * SB * SB + (i64)(i * b + tm * 64) * SB + j * b + tn * 64;
-    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[(i64)(4 * ty + r) * SB + 4 * tx + c] = -acc[r][c];
+    gemm64_mfma(A, b, B, b, b, acc, As, Bs);
+    store64(E + (i64)S * SB * SB + (i64)(i * b + tm * 64) * SB + j * b + tn * 64, SB, acc, -1.0);
 }
 
 // Et_S = E_S^T through 32 x 32 LDS tiles
