@@ -292,10 +292,18 @@ inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops, 
 // those columns.
 //   potrf(k), trsm(k, blockStart, blockCount): launch on h->stream
 //   trail(stream, k, mode, rowStart, rowCount, np): launch on `stream`
+// No work between a step's panel chain and the event that publishes it.
+struct NoPanelPrep {
+    int operator()(hipStream_t, int) const { return HS_OK; }
+};
+
 // panels = 0 takes hs_schedule_panels(n); any other value is clamped to
-// 1 .. n/2.
-template <class Potrf, class Trsm, class Trail>
-int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm trsm, Trail trail, int panels = 0) {
+// 1 .. n/2. prep(s2, k0) runs on S2 after the panels k0 .. k0+P-1 are
+// factored and before the event that releases them to S1 (the mixed factor
+// converts them to bf16 planes there, once for both streams).
+template <class Potrf, class Trsm, class Trail, class Prep = NoPanelPrep>
+int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm trsm, Trail trail, int panels = 0,
+                        Prep prep = Prep()) {
     const int P = panels > 0 ? std::max(1, std::min(panels, n / 2)) : hs_schedule_panels(n);
     int rc = hs_ensure_events(h, 2 * (size_t)n + 2);
     if (rc) return rc;
@@ -319,14 +327,14 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
         }
         return HS_OK;
     };
-    if ((rc = chain(0))) return rc;
+    if ((rc = chain(0)) || (rc = prep(s2, 0))) return rc;
     HS_HIP(hipEventRecord(evP[0], s2));
     int step = 0;
     for (int k = 0; k + P < n; k += P, ++step) {
         if (step >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[step - 1], 0));
         // columns k+P .. k+2P-1 by the step's P panels
         if ((rc = trail(s2, k, P == 1 ? 5 : 6, k + P, P == 1 ? n - k - 1 : P, P))) return rc;
-        if ((rc = chain(k + P))) return rc;
+        if ((rc = chain(k + P)) || (rc = prep(s2, k + P))) return rc;
         HS_HIP(hipEventRecord(evP[step + 1], s2));
         HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
         if (k + 2 * P < n) {

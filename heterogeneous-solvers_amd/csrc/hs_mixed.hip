@@ -473,8 +473,11 @@ __global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __r
 
 }  // namespace
 
+// planes (BF16 hi+lo only): the np panels k .. k+np-1 as bf16 hi / lo planes
+// for the block rows from r0 (split_panels_bf16_kernel layout, m = n - r0);
+// null for an update that splits the FP32 values it loads.
 int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
-                             int n, int mode, int variant, int np, unsigned short* planes) {
+                             int n, int mode, int variant, int np, const unsigned short* planes, int r0) {
     if (rowCount <= 0) return HS_OK;
     if (b % 128 != 0) {
         hs_set_error("mixed-precision trailing update needs b % 128 == 0");
@@ -491,15 +494,10 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     }
     const dim3 g((unsigned)grid), t(THREADS);
     if (variant == HS_MIXED_BF16X3) {
-        if (mode == 4 && planes) {
-            // rows and columns of a mode-4 launch start at rs = k + 2 np
-            const int rs = k + 2 * np, m = n - rs;
-            const i64 half = (i64)np * m * b * b;
-            unsigned short* Wh = planes;
-            unsigned short* Wl = planes + half;
-            const i64 sg = std::min<i64>(8192, (half / 8 + 255) / 256);
-            split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(A, b, n, k, np, rs, m, Wh, Wl);
-            HS_LAUNCH_CHECK();
+        if (planes) {
+            const int m = n - r0;
+            const unsigned short* Wh = planes;
+            const unsigned short* Wl = planes + (i64)np * m * b * b;
             if (b % T2 == 0) {
                 static bool attr = false;
                 if (!attr) {
@@ -511,9 +509,11 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
                 const i64 grid2 = hs_pair_map(k, rowStart, rowCount, n, mode, np, (i64)(b / T2) * (b / T2), &pm2);
                 if (grid2 > 0)
                     trailing_bf16x3_ps256<<<dim3((unsigned)grid2), dim3(T2THREADS), T2LDS, stream>>>(A, b, n, pm2, Wh,
-                                                                                                   Wl, rs, m);
+                                                                                                   Wl, r0, m);
+            } else if (mode >= 5) {
+                trailing_mfma_bf16x3<TILE, 1, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
             } else {
-                trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, rs, m);
+                trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
             }
         } else if (mode >= 5) {
             trailing_mfma_bf16x3<TILE, 1, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
@@ -545,6 +545,17 @@ extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int6
     return HS_OK;
 }
 
+// bf16 hi / lo planes of the np panels k .. k+np-1 for the m block rows from
+// r0 (W: hi planes, then lo planes)
+static int split_panels(hipStream_t stream, const float* L, int b, int n, int k, int np, int r0, int m,
+                        unsigned short* W) {
+    const i64 half = (i64)np * m * b * b;
+    const i64 sg = std::min<i64>(8192, (half / 8 + 255) / 256);
+    split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(L, b, n, k, np, r0, m, W, W + half);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
 // Panels per step of the mixed factor's look-ahead schedule: 6. Every rest-
 // update launch reads and writes its FP32 C tiles once, so deeper updates
 // spend less of the launch in that epilogue (BF16 hi+lo 256-tile update
@@ -568,21 +579,34 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     int rc = hs_reset_info(h);
     if (rc) return rc;
     const int P = mixed_panels(n, variant);
-    // bf16 hi / lo planes of one rest update's panels: P panels of at most n
-    // block rows, 4 bytes per element in all
+    // BF16 hi+lo: the P panels of a step as bf16 hi / lo planes for the block
+    // rows below them (4 bytes per element), split once on the panel stream
+    // when the panels are final and read by both the step's column update
+    // (S2) and its rest update (S1). Two sets alternate by step: the set of
+    // step s is rewritten at the end of step s+1's panel chain, after S2 has
+    // waited for S1's update of step s.
     unsigned short* planes = nullptr;
+    const i64 planeSet = 2 * (i64)P * n * b * b;
     if (variant == HS_MIXED_BF16X3) {
-        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)P * n * b * b))) return rc;
+        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)planeSet))) return rc;
         planes = (unsigned short*)h->work;
     }
+    auto planesOf = [&](int k) { return planes + ((k / P) & 1) * planeSet; };
+    auto prep = [&](hipStream_t s, int k0) -> int {
+        if (!planes || k0 + P >= n) return HS_OK;
+        return split_panels(s, L32, b, n, k0, P, k0 + P, n - (k0 + P), planesOf(k0));
+    };
     auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
     auto trsm = [&](int k, int start, int count) {
         return hs_factor_trsm<float>(h, L32, b, hs_block_id(k, k, n), k, start, count);
     };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
-        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np, planes);
+        // the step's rest update and its P-panel column update read the split planes
+        const bool shared = planes && np == P && (mode == 4 || (mode == 6 && P > 1));
+        return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np,
+                                        shared ? planesOf(k) : nullptr, k + P);
     };
-    if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail, P))) {
+    if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail, P, prep))) {
         hipStreamSynchronize(h->stream2);
         return rc;
     }

@@ -5,7 +5,10 @@ clocks. Every pair of launches on different streams that is not ordered by
 the events must touch disjoint blocks (no write shared with a read or a
 write), and executing the launches in an order the events allow must give
 numpy's Cholesky. Covers the panel counts the FP64 (4) and mixed (6) factors
-use and the clamping to n / 2."""
+use and the clamping to n / 2, and the mixed factor's panel planes: after a
+step's panel chain S2 copies the step's panels (rows below them) into one of
+two plane sets, which that step's P-panel column update and rest update read
+in place of the panels."""
 import itertools
 
 import numpy as np
@@ -26,7 +29,7 @@ def pairs_for(mode, k, n, np_, extra=None):
         return out
     raise ValueError
 
-def schedule(n, P):
+def schedule(n, P, planes=False):
     ops = []  # (stream, kind, args) or ('rec', stream, ev) / ('wait', stream, ev)
     S1, S2 = 1, 2
     ops.append(('rec', S1, ('U', n))); ops.append(('wait', S2, ('U', n)))
@@ -38,12 +41,15 @@ def schedule(n, P):
             if k + 1 < kend:
                 if k + 2 == kend: ops.append((S2, 'trail', (k, 5, 1)))
                 else: ops.append((S2, 'trail', (k, 6, 1, (k + 1, kend - k - 1))))
-    chain(0); ops.append(('rec', S2, ('P', 0)))
+    def prep(k0):
+        if planes and k0 + P < n:
+            ops.append((S2, 'prep', (k0,)))
+    chain(0); prep(0); ops.append(('rec', S2, ('P', 0)))
     step = 0; k = 0
     while k + P < n:
         if step >= 1: ops.append(('wait', S2, ('U', step - 1)))
         ops.append((S2, 'trail', (k, 5, 1) if P == 1 else (k, 6, P, (k + P, P))))
-        chain(k + P); ops.append(('rec', S2, ('P', step + 1)))
+        chain(k + P); prep(k + P); ops.append(('rec', S2, ('P', step + 1)))
         ops.append(('wait', S1, ('P', step)))
         if k + 2 * P < n: ops.append((S1, 'trail', (k, 4, P)))
         ops.append(('rec', S1, ('U', step)))
@@ -51,21 +57,38 @@ def schedule(n, P):
     ops.append(('wait', S1, ('P', step)))
     return ops
 
-def rw_sets(op, n):
+def plane_set(k, P):
+    return (k // P) & 1
+
+
+def shared(mode, np_, P, planes):
+    return planes and np_ == P and (mode == 4 or (mode == 6 and P > 1))
+
+
+def rw_sets(op, n, P, planes):
     kind, a = op[1], op[2]
     if kind == 'potrf': return {(a[0], a[0])}, {(a[0], a[0])}
+    if kind == 'prep':
+        k0 = a[0]
+        R = {(I, k0 + p) for I in range(k0 + P, n) for p in range(P)}
+        return R, {('W', plane_set(k0, P), I, p) for I in range(k0 + P, n) for p in range(P)}
     if kind == 'trsm':
         k, st, c = a
         return {(k, k)} | {(I, k) for I in range(st, st + c)}, {(I, k) for I in range(st, st + c)}
     k, mode, np_ = a[0], a[1], a[2]
     ps = pairs_for(mode, k, n, np_, a[3] if len(a) > 3 else None)
     R = set(ps); W = set(ps)
+    sh = shared(mode, np_, P, planes)
     for (I, J) in ps:
-        for p in range(np_): R |= {(I, k + p), (J, k + p)}
+        for p in range(np_):
+            if sh:
+                R |= {('W', plane_set(k, P), I, p), ('W', plane_set(k, P), J, p)}
+            else:
+                R |= {(I, k + p), (J, k + p)}
     return R, W
 
-def check(n, P, b=3):
-    ops = schedule(n, P)
+def check(n, P, b=3, planes=False):
+    ops = schedule(n, P, planes)
     # vector clocks: per stream op index; event -> clock snapshot
     clocks = {1: {1: 0, 2: 0}, 2: {1: 0, 2: 0}}
     ev = {}
@@ -87,7 +110,7 @@ def check(n, P, b=3):
     for a, c in itertools.combinations(real, 2):
         if a[0] == c[0]: continue
         if hb(a, c) or hb(c, a): continue
-        Ra, Wa = rw_sets(a[3], n); Rc, Wc = rw_sets(c[3], n)
+        Ra, Wa = rw_sets(a[3], n, P, planes); Rc, Wc = rw_sets(c[3], n, P, planes)
         if (Wa & (Rc | Wc)) or (Wc & Ra):
             races += 1
     # numeric: execute in program order of a topological merge (S2 first when possible)
@@ -95,6 +118,7 @@ def check(n, P, b=3):
     N = n * b
     M = rng.standard_normal((N, N)); K = M @ M.T + N * np.eye(N)
     A = np.tril(K.copy())
+    Wp = {}  # plane copies of panel blocks: (set, block row, panel) -> b x b
     blk = lambda I, J: (slice(I * b, I * b + b), slice(J * b, J * b + b))
     q = {1: [r for r in real if r[0] == 1], 2: [r for r in real if r[0] == 2]}
     pos = {1: 0, 2: 0}; order = []
@@ -114,20 +138,30 @@ def check(n, P, b=3):
         elif kind == 'trsm':
             k, st, c_ = a; L = A[blk(k, k)]
             for I in range(st, st + c_): A[blk(I, k)] = np.linalg.solve(L, A[blk(I, k)].T).T
+        elif kind == 'prep':
+            k0 = a[0]
+            for I in range(k0 + P, n):
+                for p in range(P):
+                    Wp[(plane_set(k0, P), I, p)] = A[blk(I, k0 + p)].copy()
         else:
             k, mode, np_ = a[0], a[1], a[2]
+            sh = shared(mode, np_, P, planes)
             for (I, J) in pairs_for(mode, k, n, np_, a[3] if len(a) > 3 else None):
                 for p in range(np_):
-                    A[blk(I, J)] -= A[blk(I, k + p)] @ A[blk(J, k + p)].T
+                    if sh:
+                        A[blk(I, J)] -= Wp[(plane_set(k, P), I, p)] @ Wp[(plane_set(k, P), J, p)].T
+                    else:
+                        A[blk(I, J)] -= A[blk(I, k + p)] @ A[blk(J, k + p)].T
                 if I == J: A[blk(I, J)] = np.tril(A[blk(I, J)])
     err = np.abs(np.tril(A) - np.linalg.cholesky(K)).max()
     return races, err
 
 
+@pytest.mark.parametrize("planes", [False, True])
 @pytest.mark.parametrize("P", [1, 2, 3, 4, 5, 6, 7, 8])
-def test_lookahead_schedule_is_race_free_and_exact(P):
+def test_lookahead_schedule_is_race_free_and_exact(P, planes):
     for n in range(1, 20):
         p = max(1, min(P, n // 2))
-        races, err = check(n, p)
+        races, err = check(n, p, planes=planes)
         assert races == 0, (n, p)
         assert err < 1e-8, (n, p, err)
