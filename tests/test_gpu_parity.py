@@ -167,6 +167,32 @@ def test_solve_recovers_known_solution_at_scale(hs, dev, N, b):
     assert np.all(x[N:] == 0.0)
 
 
+@pytest.mark.parametrize("N,b", [(4096, 128), (4096, 256)])
+def test_super_block_solve_backward_error_when_ill_conditioned(hs, dev, oracle, N, b):
+    """The super-block steps multiply by explicit inverses of 1024-row
+    diagonal blocks. At a noise variance of 1e-7 (cond(K) ~1e9-1e10) their
+    normwise backward error must stay at the level of the oracle's
+    substitution (the reference algorithm on the CPU)."""
+    say("ill-conditioned solve N=%d b=%d" % (N, b))
+    X, A, rhs = rbf_system(oracle, N, 8, b, sn2=1e-7)
+    Lref = oracle.cholesky(A, N, b)
+    xref = oracle.tri_solve(Lref, rhs, N, b)
+    dA, dx = dev.upload(A), dev.upload(rhs)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+    x = dev.download(dx, len(rhs))
+    K = packed_to_dense(A, N, b)
+    knorm = np.max(np.sum(np.abs(K), axis=1))
+
+    def omega(v):
+        r = rhs[:N] - K @ v[:N]
+        return np.max(np.abs(r)) / (knorm * np.max(np.abs(v[:N])) + np.max(np.abs(rhs[:N])))
+
+    w, wref = omega(x), omega(xref)
+    say("  backward error GPU %.3e, oracle %.3e" % (w, wref))
+    assert w <= max(10 * wref, 1e-15)
+
+
 def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
     N, b = 256, 64
     K = np.eye(N)
