@@ -423,8 +423,8 @@ def main():
         print(json.dumps({"calibration": "hs_vec_scale_f64", "launches": 5, "read_bytes": n * 8, "write_bytes": n * 8}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would "
-              "shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (no collectives, no sharding); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     hs = _load("hs_amd", os.path.join(PKG, "hs_amd.py"))
