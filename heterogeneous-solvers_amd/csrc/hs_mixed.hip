@@ -26,10 +26,20 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// K (FP64 packed) -> FP32 packed copy, + shift on the global diagonal
-__global__ void to_f32_kernel(const double* __restrict__ K, float* __restrict__ L, i64 total) {
-    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (i64)gridDim.x * blockDim.x)
-        L[e] = (float)K[e];
+// K (FP64 packed) -> FP32 packed copy, + shift on the global diagonal.
+// vec: four elements per thread and step (two 16-byte loads, one 16-byte
+// store) for 16-byte aligned buffers; the elements past the last multiple of
+// 4, and unaligned buffers, one at a time.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+__global__ void to_f32_kernel(const double* __restrict__ K, float* __restrict__ L, i64 total, int vec) {
+    const i64 gid = (i64)blockIdx.x * blockDim.x + threadIdx.x, stride = (i64)gridDim.x * blockDim.x;
+    const i64 q = vec ? total / 4 : 0;
+    for (i64 e = gid; e < q; e += stride) {
+        const dbl2 a = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(K) + 2 * e);
+        const dbl2 c = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(K) + 2 * e + 1);
+        reinterpret_cast<f4*>(L)[e] = f4{(float)a.x, (float)a.y, (float)c.x, (float)c.y};
+    }
+    for (i64 e = 4 * q + gid; e < total; e += stride) L[e] = (float)K[e];
 }
 
 __global__ void diag_shift_kernel(const double* __restrict__ K, float* __restrict__ L, int b, int n, i64 N,
@@ -536,7 +546,8 @@ extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int6
     if (b <= 0) return -5;
     const int n = (int)((N + b - 1) / b);
     const i64 total = (i64)n * (n + 1) / 2 * b * b;
-    to_f32_kernel<<<dim3(8192), dim3(256), 0, h->stream>>>(K, L32, total);
+    const int vec = ((uintptr_t)K % 16 == 0 && (uintptr_t)L32 % 16 == 0) ? 1 : 0;
+    to_f32_kernel<<<dim3(8192), dim3(256), 0, h->stream>>>(K, L32, total, vec);
     HS_LAUNCH_CHECK();
     if (shift != 0.0) {
         diag_shift_kernel<<<dim3((unsigned)n), dim3(256), 0, h->stream>>>(K, L32, b, n, N, shift);

@@ -22,6 +22,26 @@ def system(oracle, N, d, b, ell, sn2, seed):
     return A, rhs
 
 
+@pytest.mark.parametrize("N,b,shift", [(5, 5, 0.0), (1000, 64, 0.5), (777, 37, 1e-3), (4096, 256, 0.0)])
+def test_to_f32_packed_rounds_each_element(hs, dev, N, b, shift):
+    """The FP64 -> FP32 copy of the packed matrix (vector path for aligned
+    buffers and a length divisible by 4, element path for the rest,
+    e.g. 25 elements at N = b = 5) rounds every element and adds the shift
+    on the diagonal entries below N only."""
+    n = -(-N // b)
+    rng = np.random.default_rng(N + b)
+    A = rng.standard_normal(packed_len(N, b))
+    dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
+    assert hs.call("hs_to_f32_packed", dev.h, dA.p, dL.p, N, b, shift) == 0
+    L = dev.download(dL, packed_len(N, b), dtype=np.float32)
+    ref = A.copy()
+    for j in range(n):
+        base = (j * n - j * (j - 1) // 2) * b * b
+        for i in range(min(b, N - j * b)):
+            ref[base + i * b + i] += shift
+    assert np.array_equal(L, ref.astype(np.float32))
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("N,b", [(1024, 128), (1536, 256), (1280, 256), (600, 128), (4096, 256), (3000, 256),
                                  (4096, 128)])
