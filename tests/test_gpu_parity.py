@@ -193,6 +193,36 @@ def test_super_block_solve_backward_error_when_ill_conditioned(hs, dev, oracle, 
     assert w <= max(10 * wref, 1e-15)
 
 
+@pytest.mark.parametrize("N,b", [(3000, 64), (2300, 256), (1000, 128)])
+def test_single_direction_solves(hs, dev, oracle, N, b):
+    """hs_trsv_packed_f64: trans = 0 solves L z = y, trans = 1 solves
+    L^T x = z (against scipy on the dense oracle factor), and the two in a
+    row give hs_tri_solve_f64's result bit for bit (the same kernels)."""
+    from scipy.linalg import solve_triangular
+
+    say("single-direction solves N=%d b=%d" % (N, b))
+    X, A, rhs = rbf_system(oracle, N, 8, b, sn2=1.0)
+    dA = dev.upload(A)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    L = dev.download(dA, len(A))
+    Ld = packed_to_dense(L, N, b, lower_only=True)
+    zref = solve_triangular(Ld, rhs[:N], lower=True)
+    xref = solve_triangular(Ld.T, zref, lower=False)
+    dz = dev.upload(rhs)
+    assert hs.call("hs_trsv_packed_f64", dev.h, dA.p, dz.p, N, b, 0) == 0
+    z = dev.download(dz, len(rhs))
+    assert hs.call("hs_trsv_packed_f64", dev.h, dA.p, dz.p, N, b, 1) == 0
+    x = dev.download(dz, len(rhs))
+    dx = dev.upload(rhs)
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+    x2 = dev.download(dx, len(rhs))
+    ez = np.linalg.norm(z[:N] - zref) / np.linalg.norm(zref)
+    ex = np.linalg.norm(x[:N] - xref) / np.linalg.norm(xref)
+    say("  forward %.3e  backward %.3e" % (ez, ex))
+    assert ez <= 1e-12 and ex <= 1e-12
+    assert np.array_equal(x, x2)
+
+
 def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
     N, b = 256, 64
     K = np.eye(N)
