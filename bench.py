@@ -147,13 +147,16 @@ def cholesky_workload(hs, dev, syn, args):
         "update": {"launches": nl, "flops_per_launch": upd_flops / nl, "avg_launch_ms": float(ph[2]) / nl,
                    "tflops": achieved},
         "rel_residual": rel_res, "backward_error_inf": bwd,
+        "_x": x[:N],  # the FP64 solution, for the mixed variants' forward difference (not printed)
     }
 
 
-def mixed_workload(hs, dev, syn, args):
+def mixed_workload(hs, dev, syn, args, x64=None):
     """Mixed precision on the same N = 65536 system: time to solution of
     (FP32-stored factor + FP64 refinement to backward error 1e-12), for the
-    FP32-MFMA and the BF16 hi+lo trailing updates, K assembled beforehand."""
+    FP32-MFMA and the BF16 hi+lo trailing updates, K assembled beforehand.
+    With the FP64 path's solution x64, each variant's relative forward
+    difference from it is reported as well."""
     seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
     if args.N:
         N = args.N
@@ -192,7 +195,12 @@ def mixed_workload(hs, dev, syn, args):
         ref = float(np.median([r[1] for r in runs]))
         ph = np.mean([r[2] for r in runs], axis=0)
         _, upd_flops = mixed_update_work(hs, N, b, variant)
+        fwd = None
+        if x64 is not None:
+            xm = dev.download(dx, n * b)[:N]
+            fwd = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
         out[name] = {"factor_ms": fac, "refine_ms": ref, "solve_ms": fac + ref, "report": runs[-1][3],
+                     "forward_diff_vs_fp64": fwd,
                      "update_kernel_ms": float(ph[2]), "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
                      "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}
     dK.free()
@@ -443,7 +451,8 @@ def main():
         return
     with hs.Device(0) as dev:
         ch = cholesky_workload(hs, dev, syn, args)
-        mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args)
+        x64 = ch.pop("_x")
+        mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args, x64)
         cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
         adj = None if args.no_adjoint else adjoint_workload(hs, dev, syn, args)
         gpr = None if args.no_gpr else gpr_workload(hs, dev, syn, args)
