@@ -127,7 +127,7 @@ def test_solve_within_1e12_when_well_conditioned(hs, dev, oracle, N, b, sn2):
     assert rel <= 1e-12
 
 
-@pytest.mark.parametrize("N,b", [(40960, 256), (36864, 128)])
+@pytest.mark.parametrize("N,b", [(40960, 256), (36864, 128), (34816, 64)])
 def test_solve_recovers_known_solution_at_scale(hs, dev, N, b):
     """Sizes past the oracle's reach, where the backward super-steps near the
     bottom use their wide-grid form: y = K x_true (SYMV on the GPU), factor,
