@@ -231,11 +231,22 @@ def cg_workload(hs, dev, syn, args, symv_only=False):
     passes = (C.c_double * 2)()
     hs.call("hs_symv_packed_f64_timed", dev.h, dA.p, drhs.p, dq.p, N, b, reps, passes)
     bytes_symv = plen * 8 + 16 * n * b
+    # the same packed bytes read once by a plain streaming-read kernel: the
+    # read rate a bandwidth-bound pass over this buffer can reach
+    dpart = dev.zeros(-(-plen // 16384))
+    hs.call("hs_read_sum_f64", dev.h, dA.p, plen, dpart.p)
+    timer.start()
+    for _ in range(reps):
+        hs.call("hs_read_sum_f64", dev.h, dA.p, plen, dpart.p)
+    timer.stop()
+    read_ms = timer.ms() / reps
+    read_gbs = plen * 8 / (read_ms / 1e3) / 1e9
     if symv_only:
         dA.free()
         return {"N": N, "b": b, "symv_ms": symv_ms, "symv_gbs": bytes_symv / (symv_ms / 1e3) / 1e9,
                 "symv_block_pass_ms": passes[0], "symv_reduce_pass_ms": passes[1],
-                "symv_block_pass_gbs": bytes_symv / (passes[0] / 1e3) / 1e9}
+                "symv_block_pass_gbs": bytes_symv / (passes[0] / 1e3) / 1e9,
+                "stream_read_ms": read_ms, "stream_read_gbs": read_gbs}
     it, df, d0 = C.c_int64(), C.c_double(), C.c_double()
     timer.start()
     hs.call("hs_cg_f64", dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100000, C.byref(it), C.byref(df), C.byref(d0))
@@ -248,14 +259,17 @@ def cg_workload(hs, dev, syn, args, symv_only=False):
     rel_res = float(np.linalg.norm(rhs - dev.download(dq, n * b)) / np.linalg.norm(rhs))
     dA.free()
     log("cg workload N=%d b=%d: %d iterations in %.1f ms, symv %.3f ms (%.0f GB/s); block pass %.3f ms "
-        "(%.0f GB/s), reduction pass %.3f ms" % (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6,
-                                                 passes[0], bytes_symv / passes[0] / 1e6, passes[1]))
+        "(%.0f GB/s), reduction pass %.3f ms; plain streaming read of the same bytes %.3f ms (%.0f GB/s)"
+        % (N, b, iters, cg_ms, symv_ms, bytes_symv / symv_ms / 1e6, passes[0], bytes_symv / passes[0] / 1e6,
+           passes[1], read_ms, read_gbs))
     return {
         "N": N, "b": b, "seed": seed, "eps": 1e-10, "iterations": iters, "ms": cg_ms,
         "symv_count": nsymv, "symv_ms": symv_ms, "symv_gbs": bytes_symv / (symv_ms / 1e3) / 1e9,
         "symv_block_pass_ms": passes[0], "symv_reduce_pass_ms": passes[1],
         "symv_block_pass_gbs": bytes_symv / (passes[0] / 1e3) / 1e9,
         "symv_block_pass_hbm_frac": bytes_symv / (passes[0] / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "stream_read_ms": read_ms, "stream_read_gbs": read_gbs,
+        "symv_block_pass_vs_stream_read": read_ms / passes[0],
         "cg_gbs": nsymv * bytes_symv / (cg_ms / 1e3) / 1e9, "rel_residual_true": rel_res,
         "delta_ratio": df.value / d0.value if d0.value else 0.0,
     }

@@ -372,6 +372,39 @@ static int launch_vec(hs_handle_t h, const double* x, const double* y, double al
     return HS_OK;
 }
 
+// Streaming-read probe: work-group g sums x[g*RS_CHUNK .. +RS_CHUNK) with
+// 16-byte non-temporal loads, eight per lane in flight, and writes its sum
+// to part[g] (the sums keep the loads live; the figure of interest is the
+// read rate). vec = 0 (x not 16-byte aligned) or a partial last chunk reads
+// one element at a time.
+constexpr int RS_CHUNK = 16384;  // doubles (128 KB) per work-group
+typedef double rs_d2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void read_sum_kernel(const double* __restrict__ x, i64 count,
+                                                       double* __restrict__ part, int vec) {
+    __shared__ double red[4];
+    const i64 base = (i64)blockIdx.x * RS_CHUNK;
+    const i64 end = min(count, base + RS_CHUNK);
+    double s = 0.0;
+    if (vec && end - base == RS_CHUNK) {
+        const rs_d2* p = reinterpret_cast<const rs_d2*>(x + base);
+#pragma unroll
+        for (int r = 0; r < RS_CHUNK / 2 / 256 / 8; ++r) {
+            rs_d2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(p + (r * 8 + u) * 256 + threadIdx.x);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u].x + v[u].y;
+        }
+    } else {
+        for (i64 i = base + threadIdx.x; i < end; i += 256) s += x[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // First pass: G work-groups, each sums a contiguous chunk in a fixed order
 // (per-thread strided partial sums, then a fixed LDS tree).
 constexpr int DOT_THREADS = 256;
@@ -536,6 +569,22 @@ extern "C" int hs_vec_sub_f64(hs_handle_t h, const double* x, const double* y, d
 extern "C" int hs_vec_scale_add_f64(hs_handle_t h, const double* x, double alpha, const double* y, double* result,
                                     int b, int blockStart, int blockCount) {
     return launch_vec(h, x, y, alpha, result, b, blockStart, blockCount, 3);
+}
+
+extern "C" int hs_read_sum_f64(hs_handle_t h, const double* x, int64_t count, double* partials_dev) {
+    if (!h) return -1;
+    if (!x) return -2;
+    if (count <= 0) return -3;
+    if (!partials_dev) return -4;
+    const i64 groups = (count + RS_CHUNK - 1) / RS_CHUNK;
+    if (groups > 0x7fffffffLL) {
+        hs_set_error("hs_read_sum_f64: count too large");
+        return -3;
+    }
+    const int vec = ((uintptr_t)x % 16 == 0) ? 1 : 0;
+    read_sum_kernel<<<dim3((unsigned)groups), dim3(256), 0, h->stream>>>(x, count, partials_dev, vec);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
 }
 
 extern "C" int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount,

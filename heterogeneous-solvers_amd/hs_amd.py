@@ -64,6 +64,7 @@ _SIGS = {
     "hs_tri_solve_f64": [_P, _P, _P, _L, _I],
     "hs_cholesky_schedule_f64": [_L, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_cholesky_mixed_schedule": [_L, _I, _I, C.POINTER(_L), C.POINTER(_D)],
+    "hs_read_sum_f64": [_P, _P, _L, _P],
     "hs_cg_f64": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],
     "hs_rbf_assemble_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P],
     "hs_rbf_window_f64": [_P, _P, _L, _I, _P],

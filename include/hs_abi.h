@@ -143,6 +143,12 @@ int hs_vec_scale_add_f64(hs_handle_t h, const double* x, double alpha, const dou
  * Replaces scalarProduct + sumFinalScalarProduct (VectorOperations.cpp:84-185). */
 int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount, double* result_dev);
 
+/* Streaming-read probe: reads count doubles once (16-byte non-temporal loads,
+ * one work-group per 16384 doubles) and writes each work-group's sum to
+ * partials_dev (ceil(count / 16384) doubles). Measurement helper that gives
+ * the read rate a bandwidth-bound kernel can reach; no reference counterpart. */
+int hs_read_sum_f64(hs_handle_t h, const double* x, int64_t count, double* partials_dev);
+
 /* ---- whole-solver paths -------------------------------------------------- */
 
 /* Right-looking blocked Cholesky A -> L in place on the device, stream

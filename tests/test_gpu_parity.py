@@ -223,6 +223,21 @@ def test_single_direction_solves(hs, dev, oracle, N, b):
     assert np.array_equal(x, x2)
 
 
+@pytest.mark.parametrize("count", [16384 * 3, 16384 * 2 + 1000, 777])
+def test_read_sum_probe_reads_every_element(hs, dev, count):
+    """The streaming-read probe behind the bench's read-ceiling figure sums
+    every element once (full chunks on the vector path, a partial last chunk
+    element by element)."""
+    x = np.random.default_rng(count).standard_normal(count)
+    dx = dev.upload(x)
+    g = -(-count // 16384)
+    dp = dev.zeros(g)
+    assert hs.call("hs_read_sum_f64", dev.h, dx.p, count, dp.p) == 0
+    part = dev.download(dp, g)
+    ref = [x[i * 16384:(i + 1) * 16384].sum() for i in range(g)]
+    assert np.allclose(part, ref, rtol=1e-12, atol=1e-9)
+
+
 def test_cholesky_reports_first_bad_pivot(hs, dev, oracle):
     N, b = 256, 64
     K = np.eye(N)
