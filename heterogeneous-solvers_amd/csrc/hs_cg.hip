@@ -86,21 +86,18 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
     __shared__ double colred[16][BS];
 
     const i64 id = blockIdx.x;
-    int I, J;
-    decode_block(id, n, &I, &J);
-    const bool diag = (I == J);
     const double* blk = A + id * (i64)BS * BS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, c16 = lane & 15;
-    const double* xJ = x + (i64)J * BS;
-    const double* xI = x + (i64)I * BS;
-
+    // the block's (I, J) and the x segments are needed only for the products:
+    // they are found after the first batch's block loads are issued, so that
+    // the decode's square root and loops run under those loads
+    int I = 0, J = 0;
+    const double* xI = x;
     double xj[2 * NS], cacc[2 * NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double2 v = *reinterpret_cast<const double2*>(xJ + 2 * c16 + 32 * s);
-        xj[2 * s] = v.x;
-        xj[2 * s + 1] = v.y;
+        xj[2 * s] = xj[2 * s + 1] = 0.0;
         cacc[2 * s] = 0.0;
         cacc[2 * s + 1] = 0.0;
     }
@@ -121,6 +118,17 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
             for (int s = 0; s < NS; ++s) {
                 a[g][2 * s] = __builtin_nontemporal_load(rp + 32 * s);
                 a[g][2 * s + 1] = __builtin_nontemporal_load(rp + 32 * s + 1);
+            }
+        }
+        if (g0 == 0) {
+            decode_block(id, n, &I, &J);
+            xI = x + (i64)I * BS;
+            const double* xJ = x + (i64)J * BS;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double2 v = *reinterpret_cast<const double2*>(xJ + 2 * c16 + 32 * s);
+                xj[2 * s] = v.x;
+                xj[2 * s + 1] = v.y;
             }
         }
         double pb[GB];
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(SYMV_THREADS) void symv_partials_kernel(const doubl
         if ((c16 & (LPB - 1)) == 0)
             __builtin_nontemporal_store(tot, rowpart + id * BS + wave * RW + 4 * (g0 + c16 / LPB) + q);
     }
-    if (diag) return;  // diagonal blocks hold both triangles: row form only
+    if (I == J) return;  // diagonal blocks hold both triangles: row form only
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         colred[wave * 4 + q][2 * c16 + 32 * s] = cacc[2 * s];
