@@ -451,8 +451,12 @@ def main():
 
     hs = _load("hs_amd", os.path.join(PKG, "hs_amd.py"))
     syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
+    chol_n = args.N or syn.WORKLOADS["cholesky"][1]
+    # every system the run solves, reserved up front: no solver entry allocates
+    sizes = [(chol_n, syn.WORKLOADS["cholesky"][3]), (syn.WORKLOADS["cg"][1], syn.WORKLOADS["cg"][3], 8192),
+             (syn.WORKLOADS["adjoint"][1], syn.WORKLOADS["adjoint"][3])]
     if args.only:
-        with hs.Device(0) as dev:
+        with hs.Device(0, reserve=sizes) as dev:
             if args.only == "adjoint":
                 rec = adjoint_workload(hs, dev, syn, args)
             elif args.only == "gpr":
@@ -463,7 +467,7 @@ def main():
                 rec = cg_workload(hs, dev, syn, args, symv_only=(args.only == "symv"))
         print(json.dumps({args.only: rec}))
         return
-    with hs.Device(0) as dev:
+    with hs.Device(0, reserve=sizes) as dev:
         ch = cholesky_workload(hs, dev, syn, args)
         x64 = ch.pop("_x")
         mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args, x64)

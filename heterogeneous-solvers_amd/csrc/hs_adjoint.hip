@@ -636,6 +636,11 @@ int reverse(hs_handle_t h, const double* L, double* Ab, i64 N, int b, double* ws
 
 }  // namespace
 
+size_t hs_ws_adjoint(i64 N, int b) {
+    if (b % 64 != 0 || b > 256) return 0;  // block sizes the adjoint takes
+    return sizeof(double) * (size_t)adj_layout((int)((N + b - 1) / b), b).total;
+}
+
 extern "C" int hs_gp_nll_f64(hs_handle_t h, const double* X, int64_t N, int d, int b, double ell, double sf2,
                              double sn2, const double* y, double* K, double* nll) {
     int rc = check_args(h, X, N, d, b, ell, sf2, sn2, y, K);
@@ -643,8 +648,8 @@ extern "C" int hs_gp_nll_f64(hs_handle_t h, const double* X, int64_t N, int d, i
     if (!nll) return -11;
     const int n = (int)((N + b - 1) / b);
     const AdjWork w = adj_layout(n, b);
-    if ((rc = hs_ensure_work(h, sizeof(double) * (size_t)w.total))) return rc;
-    double* ws = (double*)h->work;
+    double* ws = nullptr;
+    if ((rc = hs_work(h, sizeof(double) * (size_t)w.total, (void**)&ws))) return rc;
     if ((rc = forward(h, X, N, d, b, ell, sf2, sn2, y, K, ws, w))) return rc;
     double v = 0.0;
     HS_HIP(hipMemcpyAsync(&v, ws + w.nll, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -662,8 +667,8 @@ extern "C" int hs_gp_nll_grad_f64(hs_handle_t h, const double* X, int64_t N, int
     if (!grad) return -13;
     const int n = (int)((N + b - 1) / b);
     const AdjWork w = adj_layout(n, b);
-    if ((rc = hs_ensure_work(h, sizeof(double) * (size_t)w.total))) return rc;
-    double* ws = (double*)h->work;
+    double* ws = nullptr;
+    if ((rc = hs_work(h, sizeof(double) * (size_t)w.total, (void**)&ws))) return rc;
     if ((rc = forward(h, X, N, d, b, ell, sf2, sn2, y, K, ws, w))) return rc;
     if ((rc = reverse(h, K, Kbar, N, b, ws, w))) return rc;
     const size_t lds = sizeof(double) * 2 * (size_t)b * d;

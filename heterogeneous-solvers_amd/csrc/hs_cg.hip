@@ -382,15 +382,24 @@ struct SymvWork {
     CGState* st;
 };
 
-static int symv_workspace(hs_handle_t h, int n, int b, bool withVectors, SymvWork* w) {
+static size_t symv_work_bytes(i64 N, int b, bool withVectors) {
+    const i64 n = (N + b - 1) / b;
+    const i64 nb = n * (n + 1) / 2;
+    const i64 rows = n * b;
+    const i64 G = rows / FIN_ROWS + 1;  // the largest partial count (reduction pass grid)
+    return sizeof(double) * (size_t)(2 * nb * b + 2 * (G + 64) + (withVectors ? 3 * rows : 0)) + sizeof(CGState) +
+           256;
+}
+
+size_t hs_ws_symv(i64 N, int b) { return symv_fast_ok(b) ? symv_work_bytes(N, b, false) : 0; }
+size_t hs_ws_cg(i64 N, int b) { return symv_fast_ok(b) ? symv_work_bytes(N, b, true) : 0; }
+
+// carves the SYMV / CG buffers out of `base` (symv_work_bytes of them)
+static void symv_carve(void* base, int n, int b, bool withVectors, SymvWork* w) {
     const i64 nb = (i64)n * (n + 1) / 2;
     const i64 rows = (i64)n * b;
-    const i64 G = rows / FIN_ROWS;  // the largest partial count (reduction pass grid)
-    const size_t bytes = sizeof(double) * (size_t)(2 * nb * b + 2 * (G + 64) + (withVectors ? 3 * rows : 0)) +
-                         sizeof(CGState) + 256;
-    int rc = hs_ensure_work(h, bytes);
-    if (rc) return rc;
-    double* p = (double*)h->work;
+    const i64 G = rows / FIN_ROWS + 1;
+    double* p = (double*)base;
     w->rowpart = p;
     p += nb * b;
     w->colpart = p;
@@ -402,6 +411,13 @@ static int symv_workspace(hs_handle_t h, int n, int b, bool withVectors, SymvWor
     w->vec = p;
     if (withVectors) p += 3 * rows;
     w->st = (CGState*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
+}
+
+static int symv_workspace(hs_handle_t h, int n, int b, bool withVectors, SymvWork* w) {
+    void* base = nullptr;
+    int rc = hs_work(h, symv_work_bytes((i64)n * b, b, withVectors), &base);
+    if (rc) return rc;
+    symv_carve(base, n, b, withVectors, w);
     return HS_OK;
 }
 
@@ -447,7 +463,7 @@ extern "C" int hs_symv_packed_f64_timed(hs_handle_t h, const double* A, const do
     int rc = symv_workspace(h, n, b, false, &w);
     if (rc) return rc;
     hipEvent_t ev[3];
-    for (int e = 0; e < 3; ++e) HS_HIP(hipEventCreate(&ev[e]));
+    for (int e = 0; e < 3; ++e) HS_HIP(hipEventCreate(&ev[e]));  // timing events of this probe entry
     double t0 = 0.0, t1 = 0.0;
     const i64 G = (i64)n * b / FIN_ROWS;
     for (int r = 0; r < reps && rc == HS_OK; ++r) {
@@ -513,9 +529,7 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
     volatile i64* pinnedStop = (volatile i64*)h->pinned;
     i64 enq = 0;
     i64 stopHost = LLONG_MAX;
-    hipEvent_t evs[2];
-    HS_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-    HS_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    hipEvent_t evs[2] = {h->events[0], h->events[1]};
     int pending = -1;  // slot of the chunk whose stop copy is in flight
     auto enqueue_chunk = [&]() -> int {
         const i64 end = std::min<i64>(enq + CHUNK, imax);
@@ -564,8 +578,6 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
         pending = slot;
         slot ^= 1;
     }
-    hipEventDestroy(evs[0]);
-    hipEventDestroy(evs[1]);
     if (rc) return rc;
     HS_HIP(hipStreamSynchronize(h->stream));
     CGState hs;

@@ -293,6 +293,34 @@ extern "C" int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b,
     return rc2 ? rc2 : rc;
 }
 
+size_t hs_ws_tri_solve(i64 N, int b) {
+    if (!hs_trsv_fast_ok(b)) return 0;
+    const int n = (int)((N + b - 1) / b);
+    return sizeof(double) * ((size_t)hs_dinv_doubles(n, b) + (size_t)n * b);
+}
+
+// the reference's per-block sequence (TriangularSystemSolver.cpp:31-82) for
+// block sizes the blocked kernels do not take
+static int tri_solve_blockwise(hs_handle_t h, const double* L, double* x, i64 N, int b, int phases) {
+    const int n = (int)((N + b - 1) / b);
+    int rc;
+    if (phases & 1) {
+        for (int j = 0; j < n; ++j) {
+            const i64 id = hs_block_id(j, j, n);
+            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 0))) return rc;
+            if (j < n - 1 && (rc = hs_launch_col_update(h, L, x, b, j + 1, n - (j + 1), j, id, n, 0))) return rc;
+        }
+    }
+    if (phases & 2) {
+        for (int j = n - 1; j >= 0; --j) {
+            const i64 id = hs_block_id(j, j, n);
+            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 1))) return rc;
+            if (j > 0 && (rc = hs_launch_col_update(h, L, x, b, 0, j, j, id, n, 1))) return rc;
+        }
+    }
+    return HS_OK;
+}
+
 // Forward and backward blocked substitution (TriangularSystemSolver.cpp:31-82).
 extern "C" int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b) {
     if (!h) return -1;
@@ -305,23 +333,50 @@ extern "C" int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64
     if (hs_trsv_fast_ok(b)) {
         // workspace: the inverted diagonal (super-)blocks, then the forward vector
         const size_t nd = (size_t)hs_dinv_doubles(n, b);
-        if ((rc = hs_ensure_work(h, sizeof(double) * (nd + (size_t)n * b)))) return rc;
-        double* Dinv = (double*)h->work;
+        double* Dinv = nullptr;
+        if ((rc = hs_work(h, hs_ws_tri_solve(N, b), (void**)&Dinv))) return rc;
         double* z = Dinv + nd;
         if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
         return hs_tri_solve_prepared<double>(h, L, Dinv, x, z, N, b);
     }
-    for (int j = 0; j < n; ++j) {
-        const i64 id = hs_block_id(j, j, n);
-        if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 0))) return rc;
-        if (j < n - 1 && (rc = hs_launch_col_update(h, L, x, b, j + 1, n - (j + 1), j, id, n, 0))) return rc;
-    }
-    for (int j = n - 1; j >= 0; --j) {
-        const i64 id = hs_block_id(j, j, n);
-        if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 1))) return rc;
-        if (j > 0 && (rc = hs_launch_col_update(h, L, x, b, 0, j, j, id, n, 1))) return rc;
-    }
+    return tri_solve_blockwise(h, L, x, N, b, 3);
+}
+
+// Prepared factor: the inverted diagonal blocks and 1024-row super-blocks of
+// L, built once into a caller buffer, so that repeated solves with the same
+// factor skip that step.
+extern "C" int hs_tri_prepare_query(int64_t N, int b, size_t* bytes) {
+    if (N <= 0) return -1;
+    if (b <= 0) return -2;
+    if (!bytes) return -3;
+    *bytes = hs_trsv_fast_ok(b) ? sizeof(double) * (size_t)hs_dinv_doubles((int)((N + b - 1) / b), b) : 0;
     return HS_OK;
+}
+
+extern "C" int hs_tri_prepare_f64(hs_handle_t h, const double* L, int64_t N, int b, double* prep) {
+    if (!h) return -1;
+    if (!L) return -2;
+    if (N <= 0) return -3;
+    if (b <= 0) return -4;
+    if (!hs_trsv_fast_ok(b)) return HS_OK;  // nothing to prepare: the block-wise solve reads L alone
+    if (!prep) return -5;
+    return hs_diag_inverse<double>(h, L, N, b, prep);
+}
+
+extern "C" int hs_tri_solve_prepared_f64(hs_handle_t h, const double* L, const double* prep, double* x, int64_t N,
+                                         int b) {
+    if (!h) return -1;
+    if (!L) return -2;
+    if (N <= 0) return -5;
+    if (b <= 0) return -6;
+    if (!x) return -4;
+    if (!hs_trsv_fast_ok(b)) return tri_solve_blockwise(h, L, x, N, b, 3);
+    if (!prep) return -3;
+    const int n = (int)((N + b - 1) / b);
+    double* z = nullptr;
+    int rc = hs_work(h, sizeof(double) * (size_t)n * b, (void**)&z);
+    if (rc) return rc;
+    return hs_tri_solve_prepared<double>(h, L, prep, x, z, N, b);
 }
 
 // One triangular solve in place on x: L x' = x (trans 0) or L^T x' = x
@@ -339,8 +394,8 @@ extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int
     if (hs_trsv_fast_ok(b)) {
         const size_t nb = (size_t)n * b;
         const size_t nd = (size_t)hs_dinv_doubles(n, b);
-        if ((rc = hs_ensure_work(h, sizeof(double) * (nd + nb)))) return rc;
-        double* Dinv = (double*)h->work;
+        double* Dinv = nullptr;
+        if ((rc = hs_work(h, hs_ws_tri_solve(N, b), (void**)&Dinv))) return rc;
         double* z = Dinv + nd;
         if ((rc = hs_diag_inverse<double>(h, L, N, b, Dinv))) return rc;
         if (trans == 0) {
@@ -352,20 +407,7 @@ extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int
         }
         return HS_OK;
     }
-    if (trans == 0) {
-        for (int j = 0; j < n; ++j) {
-            const i64 id = hs_block_id(j, j, n);
-            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 0))) return rc;
-            if (j < n - 1 && (rc = hs_launch_col_update(h, L, x, b, j + 1, n - (j + 1), j, id, n, 0))) return rc;
-        }
-    } else {
-        for (int j = n - 1; j >= 0; --j) {
-            const i64 id = hs_block_id(j, j, n);
-            if ((rc = hs_launch_trsv_block(h, L, x, N, b, j, id, 1))) return rc;
-            if (j > 0 && (rc = hs_launch_col_update(h, L, x, b, 0, j, j, id, n, 1))) return rc;
-        }
-    }
-    return HS_OK;
+    return tri_solve_blockwise(h, L, x, N, b, trans == 0 ? 1 : 2);
 }
 
 // Work of the rest-update launches of the look-ahead schedule the factorisations

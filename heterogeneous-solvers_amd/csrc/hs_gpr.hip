@@ -216,6 +216,21 @@ extern "C" int hs_gemv_rows_f64(hs_handle_t h, const double* A, const double* x,
     return HS_OK;
 }
 
+// training splits: about 2048 work-groups in all, at least TP training
+// points each, at most 65535; returns their count and size (*chunk)
+static int predict_splits(i64 N, i64 M, i64* chunk) {
+    const i64 tiles = (M + GT - 1) / GT;
+    const i64 maxS = std::max<i64>(1, (N + TP - 1) / TP);
+    const i64 S = std::min<i64>(std::min<i64>(maxS, 65535), std::max<i64>(1, (2048 + tiles - 1) / tiles));
+    *chunk = (N + S - 1) / S;
+    return (int)((N + *chunk - 1) / *chunk);
+}
+
+size_t hs_ws_gp_predict(i64 N, i64 M) {
+    i64 chunk = 0;
+    return sizeof(double) * (size_t)(M * predict_splits(N, M, &chunk));
+}
+
 extern "C" int hs_gp_predict_f64(hs_handle_t h, const double* Xtrain, int64_t N, const double* Xtest, int64_t M, int d,
                                  double ell, double sf2, const double* alpha, double* mean) {
     int rc = check_cross(h, Xtrain, N, Xtest, M, d, ell);
@@ -224,14 +239,10 @@ extern "C" int hs_gp_predict_f64(hs_handle_t h, const double* Xtrain, int64_t N,
     if (!mean) return -10;
     const i64 tiles = (M + GT - 1) / GT;
     if (tiles > 0x7fffffffLL) return -5;
-    // training splits: about 2048 work-groups in all, at least TP training
-    // points each, at most 65535
-    const i64 maxS = std::max<i64>(1, (N + TP - 1) / TP);
-    const i64 S = std::min<i64>(std::min<i64>(maxS, 65535), std::max<i64>(1, (2048 + tiles - 1) / tiles));
-    const i64 chunk = (N + S - 1) / S;
-    const int Sn = (int)((N + chunk - 1) / chunk);
-    if ((rc = hs_ensure_work(h, sizeof(double) * (size_t)(M * Sn)))) return rc;
-    double* part = (double*)h->work;
+    i64 chunk = 0;
+    const int Sn = predict_splits(N, M, &chunk);
+    double* part = nullptr;
+    if ((rc = hs_work(h, sizeof(double) * (size_t)(M * Sn), (void**)&part))) return rc;
     const double scale = -0.5 / (ell * ell);
     const dim3 grid((unsigned)tiles, (unsigned)Sn), blk(GT);
     switch (d) {

@@ -11,14 +11,18 @@
 
 typedef int64_t i64;
 
-// Scratch sizes are fixed per handle once grown; kernels never allocate.
+// The handle's device workspace is sized by hs_workspace_reserve (a set-up
+// call); solver entries take their scratch from it through hs_work and never
+// allocate.
+constexpr int HS_HANDLE_EVENTS = 8;  // ordering events created with the handle
+
 struct hs_handle_s {
     int device = 0;
     hipStream_t stream = nullptr;   // the handle's ordering stream
     hipStream_t stream2 = nullptr;  // look-ahead panel stream (same priority)
-    std::vector<hipEvent_t> events; // reusable ordering events (no timing)
+    std::vector<hipEvent_t> events; // reusable ordering events (no timing), HS_HANDLE_EVENTS at creation
     int* info_dev = nullptr;   // sticky first bad pivot (1-based global row), 0 = none
-    // grow-only device workspace for partial sums / scalars
+    // device workspace for partial sums, vectors and scalars of the entries
     void* work = nullptr;
     size_t work_bytes = 0;
     // pinned host word used to poll device-side loop state without a full copy
@@ -36,9 +40,23 @@ struct hs_event_s {
 
 void hs_set_error(const std::string& msg);
 int hs_hip_fail(hipError_t e, const char* what);
-// Ensures the handle workspace holds at least `bytes`; called only from
-// non-hot set-up points (first call of an entry on a size).
-int hs_ensure_work(hs_handle_t h, size_t bytes);
+// The first `bytes` of the handle workspace in *ptr, or HS_ERR_WORKSPACE when
+// hs_workspace_reserve has reserved less. Never allocates.
+int hs_work(hs_handle_t h, size_t bytes, void** ptr);
+// Workspace bytes each entry family needs for a system of order N in blocks
+// of b (M test points for the GP prediction); hs_workspace_query takes the
+// maximum of them.
+size_t hs_ws_symv(i64 N, int b);
+size_t hs_ws_cg(i64 N, int b);
+size_t hs_ws_cg_ref(i64 N, int b, int wg);
+size_t hs_ws_dot_ref(i64 len, int wg);
+size_t hs_ws_dot();
+size_t hs_ws_tri_solve(i64 N, int b);
+size_t hs_ws_refine(i64 N, int b);
+size_t hs_ws_mixed(i64 N, int b);
+size_t hs_ws_adjoint(i64 N, int b);
+size_t hs_ws_gp_predict(i64 N, i64 M);
+size_t hs_ws_rbf_window(i64 N);
 // Ensures the handle's event pool holds at least `count` events.
 int hs_ensure_events(hs_handle_t h, size_t count);
 
@@ -101,6 +119,11 @@ int hs_diag_inverse(hs_handle_t h, const T* L, i64 N, int b, double* Dinv);
 template <typename T>
 int hs_tri_solve_prepared(hs_handle_t h, const T* L, const double* Dinv, double* v, double* z, i64 N, int b,
                           int phases = 3);
+// the reference-order mat-vec (1 <= b <= 256), y = A x over n block rows;
+// fma selects contracted multiply-adds; stop / it gate it like the CG kernels
+bool hs_symv_ordered_ok(int b);
+int hs_launch_symv_ordered(hs_handle_t h, const double* A, const double* x, double* y, int n, int b, int fma,
+                           const i64* stop, i64 it);
 int hs_launch_trsv_block(hs_handle_t h, const double* A, double* x, i64 N, int b, int blockRow, i64 blockID, int transposed);
 int hs_launch_col_update(hs_handle_t h, const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
                          i64 blockID, int n, int transposed);

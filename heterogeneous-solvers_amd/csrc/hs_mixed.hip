@@ -585,6 +585,13 @@ static int mixed_panels(int n, int variant) {
     return std::max(1, std::min(v, n / 2));
 }
 
+// the BF16 hi+lo factor's two plane sets (the FP32 one needs no workspace)
+size_t hs_ws_mixed(i64 N, int b) {
+    const int n = (int)((N + b - 1) / b);
+    const i64 planeSet = 2 * (i64)mixed_panels(n, HS_MIXED_BF16X3) * n * b * b;
+    return sizeof(unsigned short) * 2 * (size_t)planeSet;
+}
+
 static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, PhaseTimer& pt) {
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
@@ -599,8 +606,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     unsigned short* planes = nullptr;
     const i64 planeSet = 2 * (i64)P * n * b * b;
     if (variant == HS_MIXED_BF16X3) {
-        if ((rc = hs_ensure_work(h, sizeof(unsigned short) * 2 * (size_t)planeSet))) return rc;
-        planes = (unsigned short*)h->work;
+        if ((rc = hs_work(h, sizeof(unsigned short) * 2 * (size_t)planeSet, (void**)&planes))) return rc;
     }
     auto planesOf = [&](int k) { return planes + ((k / P) & 1) * planeSet; };
     auto prep = [&](hipStream_t s, int k0) -> int {

@@ -587,6 +587,8 @@ extern "C" int hs_read_sum_f64(hs_handle_t h, const double* x, int64_t count, do
     return HS_OK;
 }
 
+size_t hs_ws_dot() { return sizeof(double) * 1024; }
+
 extern "C" int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount,
                           double* result_dev) {
     if (!h) return -1;
@@ -597,8 +599,9 @@ extern "C" int hs_dot_f64(hs_handle_t h, const double* x, const double* y, int b
     if (blockCount < 0) return -6;
     if (!result_dev) return -7;
     const int maxG = 1024;
-    int rc = hs_ensure_work(h, sizeof(double) * maxG);
+    double* part = nullptr;
+    int rc = hs_work(h, hs_ws_dot(), (void**)&part);
     if (rc) return rc;
-    return hs_launch_dot(h, x, y, (i64)blockStart * b, (i64)blockCount * b, (double*)h->work, maxG, result_dev);
+    return hs_launch_dot(h, x, y, (i64)blockStart * b, (i64)blockCount * b, part, maxG, result_dev);
 }
 

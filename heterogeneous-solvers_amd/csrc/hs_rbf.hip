@@ -147,15 +147,17 @@ extern "C" int hs_rbf_assemble_f64(hs_handle_t h, const double* X, int64_t N, in
     return hs_launch_rbf(h, X, N, d, b, ell, sf2, sn2, A);
 }
 
+size_t hs_ws_rbf_window(i64 N) { return sizeof(double) * (size_t)N * 8; }
+
 extern "C" int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* A) {
     if (!h) return -1;
     if (!t) return -2;
     if (N <= 0) return -3;
     if (b <= 0) return -4;
     if (!A) return -5;
-    int rc = hs_ensure_work(h, sizeof(double) * (size_t)N * 8);
+    double* X = nullptr;
+    int rc = hs_work(h, hs_ws_rbf_window(N), (void**)&X);
     if (rc) return rc;
-    double* X = (double*)h->work;
     window_features_kernel<<<dim3((unsigned)((N * 8 + 255) / 256)), dim3(256), 0, h->stream>>>(t, N, X);
     HS_LAUNCH_CHECK();
     return hs_launch_rbf(h, X, N, 8, b, 1.0, 1.0, 0.01, A);

@@ -142,6 +142,14 @@ struct Ctx {
 
 }  // namespace
 
+size_t hs_ws_refine(i64 N, int b) {
+    if (!hs_trsv_fast_ok(b)) return 0;
+    const i64 n = (N + b - 1) / b, rows = n * b;
+    const i64 nb = n * (n + 1) / 2;
+    const i64 G = rows / 64 + 2048 + 64;
+    return sizeof(double) * (size_t)(hs_dinv_doubles((int)n, b) + 2 * nb * b + 5 * rows + G + 16);
+}
+
 extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x,
                                int64_t N, int b, double tol, int max_ir, int max_pcg, hs_refine_report* rep) {
     if (!h) return -1;
@@ -169,10 +177,9 @@ extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32,
     const i64 nb = (i64)c.n * (c.n + 1) / 2;
     const i64 G = c.rows / 64 + 2048 + 64;
     const i64 nd = hs_dinv_doubles(c.n, b);
-    const size_t doubles = (size_t)(nd + 2 * nb * b + 5 * c.rows + G + 16);
-    int rc = hs_ensure_work(h, doubles * sizeof(double));
+    double* w = nullptr;
+    int rc = hs_work(h, hs_ws_refine(N, b), (void**)&w);
     if (rc) return rc;
-    double* w = (double*)h->work;
     c.Dinv = w;
     w += nd;
     c.rowpart = w;

@@ -48,7 +48,13 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
     if (e == hipSuccess) e = hipMemset(h->info_dev, 0, sizeof(int) * 2);
     if (e == hipSuccess) e = hipHostMalloc(&h->pinned, 4096, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&h->trsm_w, sizeof(double) * 16 * 256);
+    for (int i = 0; i < HS_HANDLE_EVENTS && e == hipSuccess; ++i) {
+        hipEvent_t ev;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) h->events.push_back(ev);
+    }
     if (e != hipSuccess) {
+        for (auto ev : h->events) hipEventDestroy(ev);
         hipStreamDestroy(h->stream2);
         hipStreamDestroy(h->stream);
         delete h;
@@ -80,30 +86,92 @@ extern "C" int hs_sync(hs_handle_t h) {
     return HS_OK;
 }
 
-int hs_ensure_work(hs_handle_t h, size_t bytes) {
-    if (bytes <= h->work_bytes) return HS_OK;
-    // growth happens between calls, never inside a launch sequence
-    HS_HIP(hipStreamSynchronize(h->stream));
-    if (h->work) HS_HIP(hipFree(h->work));
-    h->work = nullptr;
-    h->work_bytes = 0;
-    size_t sz = bytes + (bytes >> 3) + 4096;
-    hipError_t e = hipMalloc(&h->work, sz);
-    if (e != hipSuccess) {
-        h->work = nullptr;
-        hs_set_error("hs workspace allocation failed");
-        return HS_ERR_NOMEM;
+int hs_work(hs_handle_t h, size_t bytes, void** ptr) {
+    if (bytes > h->work_bytes) {
+        hs_set_error("handle workspace too small: " + std::to_string(bytes) + " bytes needed, " +
+                     std::to_string(h->work_bytes) +
+                     " reserved; reserve it at set-up with hs_workspace_reserve(handle, N, b, M)");
+        return HS_ERR_WORKSPACE;
     }
-    h->work_bytes = sz;
+    *ptr = h->work;
     return HS_OK;
 }
 
 int hs_ensure_events(hs_handle_t h, size_t count) {
+    if (h->events.size() >= count) return HS_OK;
+    hs_set_error("handle event pool too small: " + std::to_string(count) + " events needed, " +
+                 std::to_string(h->events.size()) + " created; reserve at set-up with hs_workspace_reserve");
+    return HS_ERR_WORKSPACE;
+}
+
+static int grow_events(hs_handle_t h, size_t count) {
     while (h->events.size() < count) {
         hipEvent_t e;
         HS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         h->events.push_back(e);
     }
+    return HS_OK;
+}
+
+// The largest need of any entry for a system of order N in blocks of b with
+// M test points. The dot products over block ranges are bounded by the whole
+// vector; the reference-order dot is taken at its smallest work-group (1).
+static size_t workspace_need(i64 N, int b, i64 M) {
+    size_t need = hs_ws_dot();
+    auto take = [&](size_t v) { need = v > need ? v : need; };
+    take(hs_ws_symv(N, b));
+    take(hs_ws_cg(N, b));
+    take(hs_ws_cg_ref(N, b, 1));
+    take(hs_ws_dot_ref((N + b - 1) / b * b, 1));
+    take(hs_ws_tri_solve(N, b));
+    take(hs_ws_refine(N, b));
+    take(hs_ws_mixed(N, b));
+    take(hs_ws_adjoint(N, b));
+    take(hs_ws_rbf_window(N));
+    if (M > 0) take(hs_ws_gp_predict(N, M));
+    return need;
+}
+
+extern "C" int hs_workspace_query(int64_t N, int b, int64_t M, size_t* bytes) {
+    if (N <= 0) return -1;
+    if (b <= 0) return -2;
+    if (M < 0) return -3;
+    if (!bytes) return -4;
+    *bytes = workspace_need(N, b, M);
+    return HS_OK;
+}
+
+extern "C" int hs_workspace_reserve(hs_handle_t h, int64_t N, int b, int64_t M) {
+    if (!h) return -1;
+    if (N <= 0) return -2;
+    if (b <= 0) return -3;
+    if (M < 0) return -4;
+    HS_HIP(hipSetDevice(h->device));
+    // the look-ahead factorisation orders its two streams with 2 n + 2 events
+    int rc = grow_events(h, 2 * (size_t)((N + b - 1) / b) + 2);
+    if (rc) return rc;
+    const size_t need = workspace_need(N, b, M);
+    if (need <= h->work_bytes) return HS_OK;
+    // a set-up call: waits for the handle's work before the old buffer goes
+    HS_HIP(hipStreamSynchronize(h->stream));
+    HS_HIP(hipStreamSynchronize(h->stream2));
+    if (h->work) HS_HIP(hipFree(h->work));
+    h->work = nullptr;
+    h->work_bytes = 0;
+    hipError_t e = hipMalloc(&h->work, need);
+    if (e != hipSuccess) {
+        h->work = nullptr;
+        hs_hip_fail(e, "hs_workspace_reserve");
+        return HS_ERR_NOMEM;
+    }
+    h->work_bytes = need;
+    return HS_OK;
+}
+
+extern "C" int hs_workspace_size(hs_handle_t h, size_t* bytes) {
+    if (!h) return -1;
+    if (!bytes) return -2;
+    *bytes = h->work_bytes;
     return HS_OK;
 }
 

@@ -116,6 +116,7 @@ void GaussianProcess::start() {
     if (M == 0) throw std::invalid_argument("GaussianProcess: conf::N_test (--test_size) must be positive");
     std::cout << "-- predicting at " << M << " test inputs" << std::endl;
     hs_handle_t h = gpuQueue.handle();
+    gpuQueue.reserve(N, conf::matrixBlockSize, M);
     hs::DeviceArray<double> Xtr, Xte;
     windowFeatures(h, readSeries(path_train, N), Xtr);
     windowFeatures(h, readSeries(path_test, M), Xte);

@@ -105,6 +105,7 @@ SymmetricMatrix MatrixGenerator::generateSPDMatrix(std::string& path, hs::Queue&
     SymmetricMatrix A(N, static_cast<int>(conf::matrixBlockSize), queueGPU);
     hs::DeviceArray<double> dt(N), dA(A.matrixData.size());
     hs_handle_t h = queueGPU.handle();
+    queueGPU.reserve(N, A.blockSize);
     hs::check(hs_memcpy_htod(h, dt.data(), t.data(), N * sizeof(double)), "hs_memcpy_htod");
     hs::check(hs_rbf_window_f64(h, dt.data(), static_cast<int64_t>(N), A.blockSize, dA.data()), "hs_rbf_window_f64");
     hs::check(hs_memcpy_dtoh(h, A.matrixData.data(), dA.data(), A.matrixData.size() * sizeof(double)),

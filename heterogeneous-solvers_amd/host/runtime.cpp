@@ -52,6 +52,11 @@ void Queue::wait() {
     if (handle_) check(hs_sync(handle_), "hs_sync");
 }
 
+void Queue::reserve(std::size_t N, int b, std::size_t M) {
+    check(hs_workspace_reserve(handle(), static_cast<int64_t>(N), b, static_cast<int64_t>(M)),
+          "hs_workspace_reserve");
+}
+
 std::size_t Queue::global_mem_size() const {
     if (!handle_) return 0;
     size_t fr = 0, tot = 0;

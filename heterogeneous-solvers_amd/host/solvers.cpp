@@ -72,6 +72,7 @@ void CG::solveHeterogeneous() {
         throw std::runtime_error("GPU memory not sufficient for the matrix (this build keeps it whole on the GPU)");
     }
     auto t0 = clk::now();
+    gpuQueue.reserve(A.N, A.blockSize);
     hs::DeviceArray<double> dA(A.matrixData.size()), db(b.rightHandSideData.size()), dx(b.rightHandSideData.size());
     hs::check(hs_memcpy_htod(h, dA.data(), A.matrixData.data(), bytesA), "hs_memcpy_htod");
     hs::check(hs_memcpy_htod(h, db.data(), b.rightHandSideData.data(), bytesV), "hs_memcpy_htod");
@@ -119,6 +120,7 @@ void Cholesky::solve_heterogeneous() {
     hs_handle_t h = gpuQueue.handle();
     const std::size_t bytes = A.matrixData.size() * sizeof(double);
     auto t0 = clk::now();
+    gpuQueue.reserve(A.N, A.blockSize);
     if (!A_gpu) {
         void* p = nullptr;
         hs::check(hs_malloc(&p, bytes), "hs_malloc");
@@ -166,6 +168,7 @@ double TriangularSystemSolver::solve() {
     }
     if (!A_gpu) throw std::invalid_argument("TriangularSystemSolver: no device factor (run Cholesky first)");
     hs_handle_t h = gpuQueue.handle();
+    gpuQueue.reserve(A.N, A.blockSize);
     const std::size_t bytes = b.rightHandSideData.size() * sizeof(double);
     hs::DeviceArray<double> db(b.rightHandSideData.size());
     hs::check(hs_memcpy_htod(h, db.data(), b.rightHandSideData.data(), bytes), "hs_memcpy_htod");

@@ -17,6 +17,7 @@ HS_ERR_HIP = -1000
 HS_ERR_NOMEM = -1001
 HS_ERR_UNSUPPORTED = -1002
 HS_ERR_NOT_CONVERGED = -1003
+HS_ERR_WORKSPACE = -1004
 
 _P = C.c_void_p
 _I = C.c_int
@@ -28,6 +29,9 @@ _SIGS = {
     "hs_handle_create": [C.POINTER(_P), _I],
     "hs_handle_destroy": [_P],
     "hs_sync": [_P],
+    "hs_workspace_query": [_L, _I, _L, C.POINTER(C.c_size_t)],
+    "hs_workspace_reserve": [_P, _L, _I, _L],
+    "hs_workspace_size": [_P, C.POINTER(C.c_size_t)],
     "hs_device_count": [C.POINTER(_I)],
     "hs_malloc": [C.POINTER(_P), C.c_size_t],
     "hs_free": [_P],
@@ -62,10 +66,17 @@ _SIGS = {
     "hs_cholesky_f64": [_P, _P, _L, _I],
     "hs_cholesky_f64_timed": [_P, _P, _L, _I, C.POINTER(_D), C.POINTER(_L)],
     "hs_tri_solve_f64": [_P, _P, _P, _L, _I],
+    "hs_tri_prepare_query": [_L, _I, C.POINTER(C.c_size_t)],
+    "hs_tri_prepare_f64": [_P, _P, _L, _I, _P],
+    "hs_tri_solve_prepared_f64": [_P, _P, _P, _P, _L, _I],
     "hs_cholesky_schedule_f64": [_L, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_cholesky_mixed_schedule": [_L, _I, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_read_sum_f64": [_P, _P, _L, _P],
     "hs_cg_f64": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],
+    "hs_cg_ref_order_f64": [_P, _P, _P, _P, _L, _I, _D, _L, _I, _I, _I, C.POINTER(_L), C.POINTER(_D),
+                            C.POINTER(_D)],
+    "hs_symv_ref_order_f64": [_P, _P, _P, _P, _L, _I, _I],
+    "hs_dot_ref_order_f64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "hs_rbf_assemble_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P],
     "hs_rbf_window_f64": [_P, _P, _L, _I, _P],
     "hs_to_f32_packed": [_P, _P, _P, _L, _I, _D],
@@ -175,13 +186,25 @@ class DeviceBuffer:
 
 
 class Device:
-    """One handle (device + stream + workspace)."""
+    """One handle (device + stream + workspace). `reserve` lists the systems
+    (N, b) or (N, b, M) the handle will solve: the workspace is sized for
+    them at creation, since no solver entry allocates."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, reserve=()):
         load()
         h = _P()
         call("hs_handle_create", C.byref(h), int(device))
         self.h = h
+        for sizes in reserve:
+            self.reserve(*sizes)
+
+    def reserve(self, N, b, M=0):
+        call("hs_workspace_reserve", self.h, int(N), int(b), int(M))
+
+    def workspace_bytes(self):
+        v = C.c_size_t()
+        call("hs_workspace_size", self.h, C.byref(v))
+        return v.value
 
     def close(self):
         if self.h:

@@ -32,6 +32,10 @@ public:
     bool is_gpu() const { return handle_ != nullptr; }
     hs_handle_t handle() const;
     void wait();
+    // Sizes the device workspace for a system of order N in blocks of b (M
+    // test points of a GP prediction): the set-up step before any solver
+    // call on this queue, which never allocates by itself.
+    void reserve(std::size_t N, int b, std::size_t M = 0);
     std::size_t global_mem_size() const;
     std::string name() const;
 

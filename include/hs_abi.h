@@ -23,6 +23,12 @@
  * one handle are asynchronous and ordered by that stream (this replaces the
  * reference's host wait() after every kernel); hs_sync() is the barrier.
  * A handle is not re-entrant: use one handle per host thread.
+ *
+ * Workspace: a handle's device scratch (partial sums, vectors, inverted
+ * diagonal blocks) and its event pool are sized at set-up by
+ * hs_workspace_reserve(handle, N, b, M) for the largest system the handle
+ * will see; no solver entry allocates. An entry that needs more than was
+ * reserved returns HS_ERR_WORKSPACE and does nothing.
  */
 #ifndef HS_ABI_H
 #define HS_ABI_H
@@ -39,6 +45,7 @@ extern "C" {
 #define HS_ERR_NOMEM (-1001)
 #define HS_ERR_UNSUPPORTED (-1002)
 #define HS_ERR_NOT_CONVERGED (-1003)
+#define HS_ERR_WORKSPACE (-1004)
 
 typedef struct hs_handle_s* hs_handle_t;
 typedef struct hs_event_s* hs_event_t;
@@ -49,6 +56,17 @@ typedef struct hs_event_s* hs_event_t;
 int hs_handle_create(hs_handle_t* handle, int device);
 int hs_handle_destroy(hs_handle_t handle);
 int hs_sync(hs_handle_t handle);
+/* Workspace bytes that every entry of this ABI needs at most for a system of
+ * order N in blocks of b, with M test points for hs_gp_predict_f64 (0 if
+ * unused). Host-only arithmetic. Sizes the reference's per-solver device
+ * allocations (CG.cpp:165-234, Cholesky.cpp:24-32) once, up front. */
+int hs_workspace_query(int64_t N, int b, int64_t M, size_t* bytes);
+/* Sizes the handle's workspace and event pool for (N, b, M); keeps a larger
+ * existing reservation. A set-up call: it waits for the handle's streams and
+ * may allocate. */
+int hs_workspace_reserve(hs_handle_t handle, int64_t N, int b, int64_t M);
+/* Bytes currently reserved. */
+int hs_workspace_size(hs_handle_t handle, size_t* bytes);
 const char* hs_last_error_string(void);
 int hs_device_count(int* count);
 /* Replaces sycl::malloc_device / free / queue.memcpy (Cholesky.cpp:24-32, CG.cpp:204-229). */
@@ -177,6 +195,17 @@ int hs_cholesky_mixed_schedule(int64_t N, int b, int variant, int64_t* rest_laun
  * Replaces TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105). */
 int hs_tri_solve_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b);
 
+/* Prepared factor for repeated solves with one L: hs_tri_prepare_f64 writes
+ * the inverted diagonal blocks and 1024-row super-blocks of L into `prep`
+ * (hs_tri_prepare_query bytes, caller-owned device memory; 0 bytes for block
+ * sizes other than 64, 128 and 256, where prep may be NULL), and
+ * hs_tri_solve_prepared_f64 then solves L L^T x' = x in place without
+ * rebuilding them. Together they replace TriangularSystemSolver::solve
+ * (TriangularSystemSolver.cpp:15-105) called many times on one factor. */
+int hs_tri_prepare_query(int64_t N, int b, size_t* bytes);
+int hs_tri_prepare_f64(hs_handle_t h, const double* L, int64_t N, int b, double* prep);
+int hs_tri_solve_prepared_f64(hs_handle_t h, const double* L, const double* prep, double* x, int64_t N, int b);
+
 /* One of the two substitutions, in place on x (device): trans = 0 solves
  * L x' = x, trans = 1 solves L^T x' = x. Replaces the forward or the backward
  * half of TriangularSystemSolver::solve (TriangularSystemSolver.cpp:31-56,
@@ -189,6 +218,30 @@ int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int64_t N, int
  * Replaces CG::solveHeterogeneous at share 1.0 (CG.cpp:38-163). */
 int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
               int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero);
+
+/* The same CG in the reference's own floating-point order, the parity mode:
+ * the mat-vec sums every row over block columns 0..n-1 and columns 0..b-1 in
+ * order (matrixVectorBlock_GPU, MatrixVectorOperations.cpp:95-192), the dot
+ * products are scalarProduct's pair-load and halving tree over work-groups of
+ * wg items followed by sumFinalScalarProduct's tree over wgf items
+ * (VectorOperations.cpp:84-185; the reference's configuration has wg = 128,
+ * wgf = 512, both powers of two up to 1024), the updates are
+ * scaleAndAddVectorBlock / subVectorBlock (:45-82), in CG.cpp:38-163's step
+ * order. fma = 1 contracts `acc += a * b` and `x + y * a` into fused
+ * multiply-adds, fma = 0 rounds every product and sum. 1 <= b <= 256. More
+ * than 2 wgf dot partials (N > 4 wg wgf) is the reference's runtime_error:
+ * HS_ERR_UNSUPPORTED. Replaces CG::solveHeterogeneous at share 1.0. */
+int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                        int64_t imax, int wg, int wgf, int fma, int64_t* iterations, double* delta_final,
+                        double* delta_zero);
+/* The parity mode's mat-vec alone, y = A x over all n block rows
+ * (matrixVectorBlock_GPU over the whole matrix, reset = true). */
+int hs_symv_ref_order_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b, int fma);
+/* The parity mode's dot product over blocks [blockStart, blockStart +
+ * blockCount) into result_dev[0] (scalarProduct + sumFinalScalarProduct,
+ * VectorOperations.cpp:84-185). */
+int hs_dot_ref_order_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount,
+                         int wg, int wgf, int fma, double* result_dev);
 
 /* RBF kernel matrix into the packed layout on the device:
  * K_ij = sf2 * exp(-||x_i - x_j||^2 / (2 ell^2)) + sn2 [i == j],

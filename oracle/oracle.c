@@ -26,6 +26,17 @@
  *    dense copies, and L L^T = A reconstruction;
  *  - or_cg: closed-form solution of a dense solve (numpy) and the iteration
  *    count of a textbook CG written independently in the tests;
+ *  - or_dot_tree: the reference's dot-product goldens, exact sums of
+ *    integers, and hand-derived results on inputs where the pair-load /
+ *    halving-tree order decides the rounding (so that another work-group
+ *    size, a sequential sum or an exactly rounded sum each fail);
+ *  - or_symv_ordered: bit-for-bit equality with or_symv (fma = 0), which the
+ *    mat-vec goldens pin, and a numpy product within the rounding bound
+ *    (fma = 1);
+ *  - or_cg_gpu_order: the textbook CG of the tests, the closed-form dense
+ *    solution, and, at fma = 0, a loop in the tests that composes the pinned
+ *    or_symv, or_dot_tree, or_sub and or_scale_add in CG.cpp's step order
+ *    (bit for bit);
  *  - or_rbf / or_rbf_window: numpy brute force of the textbook RBF formula;
  *  - or_rbf_cross / or_window_features / or_gemv_gp: numpy brute force of the
  *    formula and of the window construction, K* = sf2 exactly where a test
@@ -366,6 +377,160 @@ i64 or_cg(const double* A, const double* rhs, double* x, i64 N, int b, double ep
         delta_new = or_dot(r, r, b, 0, n, chunks);
         const double beta = delta_new / delta_old;
         or_scale_add(r, beta, d, d, b, 0, n);
+        ++it;
+    }
+    if (delta_out) *delta_out = delta_new;
+    free(r);
+    free(d);
+    free(q);
+    return it;
+}
+
+/* ---------------------------------------------------------------------------
+ * The reference's share-1.0 GPU CG path, in its own arithmetic order.
+ *
+ * At share 1.0 the reference runs CG entirely on the GPU queue
+ * (src/solver/cg/CG.cpp:38-163 with blockCountGPU = n) and its kernels fix
+ * the order of every floating-point operation:
+ *  - q = A d and the true residual use matrixVectorBlock_GPU
+ *    (src/kernel/MatrixVectorOperations.cpp:95-192): one work item per row,
+ *    resultValue = 0 then += over block columns J = 0..n-1 in order and
+ *    columns j = 0..b-1 inside each block, the stored block read row-wise
+ *    for J <= I and block (J, I) read transposed for J > I. initCG uses
+ *    matrixVectorBlock (:8-93), which has the same per-row order.
+ *  - dot products use scalarProduct (src/kernel/VectorOperations.cpp:84-139):
+ *    a work-group of wg = conf::workGroupSizeVector (128) items covers 2 wg
+ *    consecutive elements; item l starts from 0, adds x[l] y[l], then
+ *    x[l + wg] y[l + wg], and the work-group sums its wg values with the
+ *    halving tree cache[l] += cache[l + s], s = wg/2 .. 1. Then
+ *    sumFinalScalarProduct (:141-185) does the same with
+ *    wgf = conf::workGroupSizeFinalScalarProduct (512) items over the
+ *    partials (and throws when there are more than 2 wgf of them).
+ *  - updates use scaleAndAddVectorBlock (x + y a, :64-82) and subVectorBlock
+ *    (x - y, :45-62); r = r - alpha q is scaleAndAdd with -1.0 * alpha.
+ * The only freedom left is whether the compiler contracts `acc += a * b`
+ * and `x + y * a` into one fused multiply-add; the reference's CMakeLists.txt
+ * sets no floating-point flag. fma = 0 evaluates every product and sum
+ * separately (this file is built with -ffp-contract=off); fma = 1 contracts
+ * exactly those three statement forms (the matrix-vector accumulation, the
+ * dot product's second pair term, the scale-and-add) with fma().
+ * ------------------------------------------------------------------------- */
+static inline double mac(double a, double x, double acc, int fma_mode) {
+    return fma_mode ? fma(a, x, acc) : acc + a * x;
+}
+
+double or_dot_tree(const double* x, const double* y, int b, int blockStart, int blockCount, int wg, int wgf,
+                   int fma_mode) {
+    const i64 len = (i64)blockCount * b;
+    const i64 off = (i64)blockStart * b;
+    const i64 half = len / 2;
+    const i64 groups = (half + wg - 1) / wg;
+    if (groups > 2 * (i64)wgf) return NAN; /* the reference throws here */
+    double* part = (double*)calloc(groups > 0 ? groups : 1, sizeof(double));
+    double* cache = (double*)malloc(sizeof(double) * (wg > wgf ? wg : wgf));
+    for (i64 g = 0; g < groups; ++g) {
+        for (int l = 0; l < wg; ++l) {
+            const i64 gi = off + g * 2 * wg + l;
+            double c = 0.0;
+            if (gi < off + len) c = mac(x[gi], y[gi], c, fma_mode);
+            if (gi + wg < off + len) c = mac(x[gi + wg], y[gi + wg], c, fma_mode);
+            cache[l] = c;
+        }
+        for (int st = wg / 2; st > 0; st /= 2)
+            for (int l = 0; l < st; ++l) cache[l] = cache[l] + cache[l + st];
+        part[g] = cache[0];
+    }
+    for (int l = 0; l < wgf; ++l) {
+        double c = 0.0;
+        if (l < groups) c = c + part[l];
+        if (l + wgf < groups) c = c + part[l + wgf];
+        cache[l] = c;
+    }
+    for (int st = wgf / 2; st > 0; st /= 2)
+        for (int l = 0; l < st; ++l) cache[l] = cache[l] + cache[l + st];
+    const double r = cache[0];
+    free(cache);
+    free(part);
+    return r;
+}
+
+/* y = A x over all n block rows, each row summed in matrixVectorBlock_GPU's
+ * order. The loops run block row by block row with the b row sums of a block
+ * row kept side by side, so that every stored block is streamed once per
+ * block row that needs it; each row's sequence of operations is exactly the
+ * per-row order above (pinned bit for bit against or_symv by the tests). */
+void or_symv_ordered(const double* A, const double* x, double* y, int b, int n, int fma_mode) {
+#pragma omp parallel for schedule(dynamic)
+    for (int I = 0; I < n; ++I) {
+        double* acc = y + (i64)I * b;
+        for (int i = 0; i < b; ++i) acc[i] = 0.0;
+        for (int J = 0; J <= I; ++J) {
+            const double* blk = A + or_block_id(I, J, n) * b * b;
+            const double* xJ = x + (i64)J * b;
+            for (int i = 0; i < b; ++i) {
+                double a = acc[i];
+                const double* row = blk + (i64)i * b;
+                for (int j = 0; j < b; ++j) a = mac(row[j], xJ[j], a, fma_mode);
+                acc[i] = a;
+            }
+        }
+        for (int J = I + 1; J < n; ++J) {
+            const double* blk = A + or_block_id(J, I, n) * b * b;
+            const double* xJ = x + (i64)J * b;
+            for (int j = 0; j < b; ++j) {
+                const double xj = xJ[j];
+                const double* col = blk + (i64)j * b;
+                for (int i = 0; i < b; ++i) acc[i] = mac(col[i], xj, acc[i], fma_mode);
+            }
+        }
+    }
+}
+
+static void axpy_ordered(const double* x, double a, const double* y, double* r, i64 len, int fma_mode) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < len; ++i) r[i] = fma_mode ? fma(y[i], a, x[i]) : x[i] + y[i] * a;
+}
+
+/* CG at share 1.0 on the GPU queue, CG.cpp:38-163 with initCG :277-340,
+ * compute_q :342-398 (gpuOptimizationLevel >= 1: matrixVectorBlock_GPU),
+ * compute_alpha :400-431, update_x :433-442, computeRealResidual at
+ * iteration % 50 == 0 :444-473, update_r :475-484, compute_delta_new
+ * :486-517, compute_d :519-528; stop on delta_new > eps^2 delta_0 failing
+ * (:62, :78); x starts at zero. wg / wgf are the two work-group sizes of the
+ * dot product (128 and 512 in the reference's configuration). Returns the
+ * iteration count, or -1 when the dot product's partial count exceeds 2 wgf
+ * (the reference's runtime_error). */
+i64 or_cg_gpu_order(const double* A, const double* rhs, double* x, i64 N, int b, double eps, i64 imax, int wg,
+                    int wgf, int fma_mode, double* delta_out) {
+    const int n = (int)((N + b - 1) / b);
+    const i64 len = (i64)n * b;
+    double* r = (double*)calloc(len, sizeof(double));
+    double* d = (double*)calloc(len, sizeof(double));
+    double* q = (double*)calloc(len, sizeof(double));
+    memset(x, 0, sizeof(double) * len);
+    or_symv_ordered(A, x, r, b, n, fma_mode);
+    or_sub(rhs, r, r, b, 0, n);
+    memcpy(d, r, sizeof(double) * len);
+    double delta_new = or_dot_tree(r, r, b, 0, n, wg, wgf, fma_mode);
+    i64 it = 0;
+    if (isnan(delta_new)) it = -1;
+    const double delta0 = delta_new;
+    const double eps2 = eps * eps;
+    while (it >= 0 && it < imax && delta_new > eps2 * delta0) {
+        or_symv_ordered(A, d, q, b, n, fma_mode);
+        const double dq = or_dot_tree(d, q, b, 0, n, wg, wgf, fma_mode);
+        const double alpha = delta_new / dq;
+        axpy_ordered(x, alpha, d, x, len, fma_mode);
+        if (it % 50 == 0) {
+            or_symv_ordered(A, x, r, b, n, fma_mode);
+            or_sub(rhs, r, r, b, 0, n);
+        } else {
+            axpy_ordered(r, -1.0 * alpha, q, r, len, fma_mode);
+        }
+        const double delta_old = delta_new;
+        delta_new = or_dot_tree(r, r, b, 0, n, wg, wgf, fma_mode);
+        const double beta = delta_new / delta_old;
+        axpy_ordered(r, beta, d, d, len, fma_mode);
         ++it;
     }
     if (delta_out) *delta_out = delta_new;

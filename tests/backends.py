@@ -8,8 +8,9 @@ from hs_testlib import Oracle, dptr
 
 
 class OracleBackend:
-    def __init__(self, oracle=None):
+    def __init__(self, oracle=None, fma=0):
         self.o = oracle or Oracle()
+        self.fma = fma
 
     def begin(self, arrays):
         self.a = {k: np.array(v, dtype=np.float64) for k, v in arrays.items()}
@@ -52,12 +53,16 @@ class OracleBackend:
     def dot(self, x, y, res, b, bs, bc):
         self.a[res][0] = self.o.or_dot(dptr(self.a[x]), dptr(self.a[y]), b, bs, bc, self.o.or_num_threads())
 
+    def dot_tree(self, x, y, res, b, bs, bc, wg):
+        self.a[res][0] = self.o.or_dot_tree(dptr(self.a[x]), dptr(self.a[y]), b, bs, bc, wg, 512, self.fma)
+
 
 class HipBackend:
     """Runs every op through libhs_amd.so on the GPU."""
 
-    def __init__(self, hs, dev):
+    def __init__(self, hs, dev, fma=0):
         self.hs, self.dev = hs, dev
+        self.fma = fma
 
     def begin(self, arrays):
         self.host = {k: np.array(v, dtype=np.float64) for k, v in arrays.items()}
@@ -107,3 +112,7 @@ class HipBackend:
 
     def dot(self, x, y, res, b, bs, bc):
         self._c("hs_dot_f64", self.buf[x].p, self.buf[y].p, b, bs, bc, self.buf[res].p)
+
+    def dot_tree(self, x, y, res, b, bs, bc, wg):
+        self._c("hs_dot_ref_order_f64", self.buf[x].p, self.buf[y].p, b, bs, bc, wg, 512, self.fma,
+                self.buf[res].p)

@@ -39,6 +39,9 @@ def dev(hs):
     rc = lib.hs_device_count(C.byref(cnt))
     if rc != 0 or cnt.value == 0:
         pytest.skip("no GPU visible to the HIP runtime")
-    d = hs.Device(0)
+    # the largest systems the GPU tests solve on the shared handle: the
+    # workspace and event pool are sized once here, as a caller would at
+    # set-up (no solver entry allocates)
+    d = hs.Device(0, reserve=[(65536, 64), (65536, 128), (65536, 256, 16384), (4096, 1)])
     yield d
     d.close()

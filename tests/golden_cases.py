@@ -41,7 +41,7 @@ OPS = {
     "VectorOperations::addVectorBlock": "add",
     "VectorOperations::subVectorBlock": "sub",
     "VectorOperations::scaleAndAddVectorBlock": "scale_add",
-    "VectorOperations::scalarProduct": "dot",
+    "VectorOperations::scalarProduct": "dot_tree",
     "VectorOperations::scalarProduct_CPU": "dot",
     "VectorOperations::sumFinalScalarProduct": None,
     "VectorOperations::sumFinalScalarProduct_CPU": None,
@@ -128,8 +128,12 @@ def setup_case(t):
 
 
 def run_case(t, backend):
-    """Runs the calls of case t on `backend`, returns {name: final array}."""
+    """Runs the calls of case t on `backend`, returns {name: final array}.
+    scalarProduct (the GPU kernel, VectorOperations.cpp:84-139) runs as the
+    work-group tree of the case's conf::workGroupSizeVector; the final
+    reduction's work-group size is the configuration default (512)."""
     b, N, env, arrays = setup_case(t)
+    wg = t["config"].get("workGroupSizeVector", 128)
     backend.begin(arrays)
     for fname, args in t["calls"]:
         op = OPS[fname]
@@ -159,6 +163,8 @@ def run_case(t, backend):
             backend.vec("scale_add", bufs[0], bufs[2], float(nums[1]), bufs[3], b, int(nums[4]), int(nums[5]))
         elif op == "dot":
             backend.dot(bufs[0], bufs[1], bufs[2], b, int(nums[3]), int(nums[4]))
+        elif op == "dot_tree":
+            backend.dot_tree(bufs[0], bufs[1], bufs[2], b, int(nums[3]), int(nums[4]), wg)
     return backend.end(), env
 
 

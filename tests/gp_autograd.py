@@ -19,10 +19,13 @@ hs = load_hs()
 _dev = None
 
 
-def device():
+def device(N, b):
+    """The harness's handle, its workspace reserved for (N, b) before the
+    call (a set-up step; a no-op once reserved)."""
     global _dev
     if _dev is None:
         _dev = hs.Device(0)
+    _dev.reserve(N, b)
     return _dev
 
 
@@ -40,7 +43,7 @@ def nll_only(theta, X, y, b):
     out = C.c_double()
     ell, sf2, sn2 = (math.exp(float(t)) for t in theta)
     torch.cuda.synchronize()
-    rc = hs.call("hs_gp_nll_f64", device().h, C.c_void_p(X.data_ptr()), N, d, b, ell, sf2, sn2,
+    rc = hs.call("hs_gp_nll_f64", device(N, b).h, C.c_void_p(X.data_ptr()), N, d, b, ell, sf2, sn2,
                  C.c_void_p(y.data_ptr()), C.c_void_p(K.data_ptr()), C.byref(out))
     if rc:
         raise RuntimeError("kernel matrix not positive definite at row %d" % rc)
@@ -54,7 +57,7 @@ def nll_grad(theta, X, y, b, want_kbar=False):
     g = (C.c_double * 3)()
     ell, sf2, sn2 = (math.exp(float(t)) for t in theta)
     torch.cuda.synchronize()
-    rc = hs.call("hs_gp_nll_grad_f64", device().h, C.c_void_p(X.data_ptr()), N, d, b, ell, sf2, sn2,
+    rc = hs.call("hs_gp_nll_grad_f64", device(N, b).h, C.c_void_p(X.data_ptr()), N, d, b, ell, sf2, sn2,
                  C.c_void_p(y.data_ptr()), C.c_void_p(K.data_ptr()), C.c_void_p(Kb.data_ptr()), C.byref(out), g)
     if rc:
         raise RuntimeError("kernel matrix not positive definite at row %d" % rc)

@@ -67,6 +67,9 @@ class Oracle:
             "or_rbf_cross": ([_P, _L, _P, _L, _I, _D, _D, _P], None),
             "or_gemv_gp": ([_P, _P, _P, _L, _L], None),
             "or_num_threads": ([], _I),
+            "or_dot_tree": ([_P, _P, _I, _I, _I, _I, _I, _I], _D),
+            "or_symv_ordered": ([_P, _P, _P, _I, _I, _I], None),
+            "or_cg_gpu_order": ([_P, _P, _P, _L, _I, _D, _L, _I, _I, _I, C.POINTER(_D)], _L),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -93,6 +96,23 @@ class Oracle:
         delta = _D()
         it = self.L.or_cg(dptr(A), dptr(rhs), dptr(x), N, b, eps, imax, chunks, C.byref(delta))
         return x, it, delta.value
+
+    def cg_gpu_order(self, A, rhs, N, b, eps, imax=100000, fma=0, wg=128, wgf=512):
+        """CG in the order of the reference's share-1.0 GPU path."""
+        x = np.zeros_like(rhs)
+        delta = _D()
+        it = self.L.or_cg_gpu_order(dptr(A), dptr(rhs), dptr(x), N, b, eps, imax, wg, wgf, fma, C.byref(delta))
+        return x, it, delta.value
+
+    def dot_tree(self, x, y, b, fma=0, wg=128, wgf=512, block_start=0, block_count=None):
+        n = len(x) // b
+        return self.L.or_dot_tree(dptr(x), dptr(y), b, block_start, n if block_count is None else block_count,
+                                  wg, wgf, fma)
+
+    def symv_ordered(self, A, x, b, n, fma=0):
+        y = np.zeros(n * b)
+        self.L.or_symv_ordered(dptr(A), dptr(x), dptr(y), b, n, fma)
+        return y
 
     def symv(self, A, x, b, n):
         y = np.zeros(n * b)
