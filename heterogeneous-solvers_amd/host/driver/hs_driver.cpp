@@ -50,6 +50,9 @@ int main(int argc, char** argv) {
         if (a.count("print_verbose")) conf::printVerbose = toBool(a["print_verbose"]);
         if (a.count("check_result")) conf::checkResult = toBool(a["check_result"]);
         if (a.count("test_size")) conf::N_test = std::stoul(a["test_size"]);
+        // this build's flags: CG in the reference's floating-point order
+        if (a.count("reference_order")) conf::referenceOrder = toBool(a["reference_order"]);
+        if (a.count("fma")) conf::referenceOrderFMA = toBool(a["fma"]);
         std::string gpTest = a.count("gp_test") ? a["gp_test"] : "";
         const bool performGPR = a.count("gpr") && toBool(a["gpr"]);
         const bool generate = !gpIn.empty();

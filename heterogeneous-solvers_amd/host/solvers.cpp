@@ -81,9 +81,17 @@ void CG::solveHeterogeneous() {
     t0 = clk::now();
     int64_t it = 0;
     double df = 0.0, d0 = 0.0;
-    hs::check(hs_cg_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize, conf::epsilon,
-                        static_cast<int64_t>(conf::iMax), &it, &df, &d0),
-              "hs_cg_f64");
+    if (conf::referenceOrder) {
+        hs::check(hs_cg_ref_order_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize,
+                                      conf::epsilon, static_cast<int64_t>(conf::iMax), conf::workGroupSizeVector,
+                                      conf::workGroupSizeFinalScalarProduct, conf::referenceOrderFMA ? 1 : 0, &it,
+                                      &df, &d0),
+                  "hs_cg_ref_order_f64");
+    } else {
+        hs::check(hs_cg_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize,
+                            conf::epsilon, static_cast<int64_t>(conf::iMax), &it, &df, &d0),
+                  "hs_cg_f64");
+    }
     const double solveMs = ms_since(t0);
     iterations = static_cast<std::size_t>(it);
     deltaZero = d0;

@@ -37,6 +37,12 @@ struct Configuration {
     bool printVerbose = false;
     bool checkResult = false;
     bool trackCholeskySolveStep = true;
+    // Keys of this build (no reference counterpart): CG in the reference's own
+    // floating-point order (its GPU kernels' summation order, the
+    // work-group sizes above), so that iteration counts compare with the
+    // reference's GPU build; and whether that order contracts multiply-adds.
+    bool referenceOrder = false;
+    bool referenceOrderFMA = true;
 };
 
 Configuration& get();
@@ -70,4 +76,6 @@ inline int& cpuOptimizationLevel = get().cpuOptimizationLevel;
 inline bool& printVerbose = get().printVerbose;
 inline bool& checkResult = get().checkResult;
 inline bool& trackCholeskySolveStep = get().trackCholeskySolveStep;
+inline bool& referenceOrder = get().referenceOrder;
+inline bool& referenceOrderFMA = get().referenceOrderFMA;
 }  // namespace conf

@@ -52,6 +52,10 @@ def _gp_files(tmp_path, N, seed=7):
 
 
 def test_driver_cg_on_gp_input_files(tmp_path, oracle):
+    """The fast read-once CG through the driver on window-feature inputs: the
+    matrix is assembled on the GPU (a few ulp from the oracle's), the
+    summation order is the fast path's own, so the count is held to 5% and the
+    solution to the convergence tolerance."""
     N, b = 1500, 64
     t, y = _gp_files(tmp_path, N)
     r = drive(tmp_path, "--gp_input", tmp_path / "gp_in.txt", "--gp_output", tmp_path / "gp_out.txt", "--size", N,
@@ -68,6 +72,50 @@ def test_driver_cg_on_gp_input_files(tmp_path, oracle):
     print("iterations driver %d oracle %d" % (it, itref))
     assert abs(it - itref) <= max(1, itref // 20)
     assert np.linalg.norm(x - xref) <= 1e-6 * np.linalg.norm(xref)
+
+
+def _write_matrix_text(path, K):
+    """The reference's matrix format (MatrixParser.cpp:11-72): '# N', then
+    row i's entries 0..i separated by ';', 17 significant digits (exact)."""
+    N = K.shape[0]
+    with open(path, "w") as fh:
+        fh.write("# %d\n" % N)
+        for i in range(N):
+            fh.write(";".join("%.17g" % v for v in K[i, :i + 1]) + "\n")
+
+
+def _write_rhs_text(path, y):
+    with open(path, "w") as fh:
+        fh.write("# %d\n" % len(y))
+        fh.write(";".join("%.17g" % v for v in y) + "\n")
+
+
+@pytest.mark.parametrize("fma", [0, 1])
+def test_driver_cg_reference_order_iterations(tmp_path, oracle, fma):
+    """--reference_order: the host CG runs the reference's GPU arithmetic
+    order; on the oracle's own matrix (written in the reference's text format,
+    exact to the bit) the iteration count is the oracle's within +-1."""
+    from hs_testlib import packed_to_dense, synth_gp_numpy
+
+    N, b = 1500, 64
+    X, y = synth_gp_numpy(9, N, 8)
+    A = oracle.rbf(X, N, 8, b, 1.0, 1.0, 1e-2)
+    _write_matrix_text(tmp_path / "A.txt", packed_to_dense(A, N, b))
+    _write_rhs_text(tmp_path / "b.txt", y)
+    r = drive(tmp_path, "--path_A", tmp_path / "A.txt", "--path_b", tmp_path / "b.txt", "--matrix_bsz", b,
+              "--init_gpu_perc", 1, "--algorithm", "cg", "--eps", 1e-10, "--reference_order", "true",
+              "--fma", "true" if fma else "false", "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    xref, itref, _ = oracle.cg_gpu_order(A, rhs, N, b, 1e-10, fma=fma)
+    it = int(result(r, "iterations"))
+    print("reference order fma=%d: iterations driver %d oracle %d" % (fma, it, itref), flush=True)
+    assert abs(it - itref) <= 1
+    x = np.loadtxt(tmp_path / "x_result.txt")
+    if it == itref:
+        assert np.linalg.norm(x - xref) <= 1e-12 * np.linalg.norm(xref)
 
 
 def test_driver_cholesky_on_gp_input_files(tmp_path, oracle):

@@ -87,8 +87,14 @@ def test_potrf_and_trsm_match_oracle(hs, dev, oracle, b):
                                  # two-panel look-ahead edge cases: 4, 5 and 7 block columns, padded or not
                                  (512, 128), (1280, 256), (1100, 256), (448, 64), (330, 64),
                                  # 1024-row super-block steps: padded last super-block, remainder blocks
-                                 (2000, 128), (2300, 256), (3000, 64)])
+                                 (2000, 128), (2300, 256), (3000, 64),
+                                 # the Cholesky + solve workload on the CPU reference path, exactly:
+                                 # N = 4096, b = 128, d = 8, seed 1, l = 1, sf2 = 1, sn2 = 1e-2
+                                 (4096, 128)])
 def test_cholesky_and_solve_match_oracle(hs, dev, oracle, N, b):
+    """sn2 = 1e-2, d = 8 (cond(K) up to ~1e5): the factor within the
+    factorisations' rounding, the solution within 1e-12 relative of the
+    oracle's (the reference algorithm on the CPU)."""
     say("cholesky+solve N=%d b=%d" % (N, b))
     X, A, rhs = rbf_system(oracle, N, 8, b)
     Lref = oracle.cholesky(A, N, b)
@@ -107,7 +113,48 @@ def test_cholesky_and_solve_match_oracle(hs, dev, oracle, N, b):
     rel_diff = np.linalg.norm(x - xref) / np.linalg.norm(xref)
     say("  rel residual %.3e  rel diff vs oracle %.3e" % (rel_res, rel_diff))
     assert rel_res <= 1e-12
-    assert rel_diff <= 1e-9
+    assert rel_diff <= 1e-12
+    assert np.all(x[N:] == 0.0)
+
+
+@pytest.mark.parametrize("N", [4096, 6144])
+def test_long_lengthscale_construction_matches_oracle(hs, dev, oracle, N):
+    """The N = 65536 workload's construction (d = 16, b = 256, l = 2,
+    sn2 = 1e-3, seed 3) at sizes the oracle finishes in seconds. Its
+    condition number bound kappa <= ||K||_inf / sn2 (the RBF part is positive
+    semi-definite, so lambda_min >= sn2) decides the bar: 1e-12 relative to
+    the oracle's solution where kappa u <= 1e-13; otherwise the normwise
+    backward error within 4x the oracle's and the difference within kappa u."""
+    b = 256
+    say("long-lengthscale construction N=%d" % N)
+    X, y = synth_gp_numpy(3, N, 16)
+    A = oracle.rbf(X, N, 16, b, 2.0, 1.0, 1e-3)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
+    dA, dx = dev.upload(A), dev.upload(rhs)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx.p, N, b)
+    x = dev.download(dx, len(rhs))
+    K = packed_to_dense(A, N, b)
+    knorm = np.max(np.sum(np.abs(K), axis=1))
+    kappa = knorm / 1e-3
+    u = np.finfo(float).eps / 2
+
+    def omega(v):
+        r = rhs[:N] - K @ v[:N]
+        return np.max(np.abs(r)) / (knorm * np.max(np.abs(v[:N])) + np.max(np.abs(rhs[:N])))
+
+    rel = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    w, wref = omega(x), omega(xref)
+    say("  kappa bound %.3e (kappa u %.2e); rel diff vs oracle %.3e; backward error GPU %.3e oracle %.3e"
+        % (kappa, kappa * u, rel, w, wref))
+    if kappa * u <= 1e-13:
+        assert rel <= 1e-12
+    else:
+        assert w <= max(4 * wref, 4 * u)
+        assert rel <= kappa * u
     assert np.all(x[N:] == 0.0)
 
 

@@ -58,16 +58,22 @@ def test_workspace_unchanged_across_solver_calls(hs, oracle, N, b):
 
 
 def test_entry_beyond_reservation_fails_without_allocating(hs, oracle):
-    N, b = 1024, 128
+    # reserved for one block (N = 128), called at N = 8192: the mat-vec
+    # partials alone need more than every entry's need at N = 128
+    N, b = 8192, 128
+    small, big = C.c_size_t(), C.c_size_t()
+    hs.call("hs_workspace_query", 128, b, 0, C.byref(small))
+    hs.call("hs_workspace_query", N, b, 0, C.byref(big))
+    assert big.value > 4 * small.value
     X, A, rhs = system(oracle, N, 8, b)
-    with hs.Device(0, reserve=[(512, b)]) as dev:
+    with hs.Device(0, reserve=[(128, b)]) as dev:
         w0 = dev.workspace_bytes()
         dA, drhs, dx = dev.upload(A), dev.upload(rhs), dev.zeros(len(rhs))
         it = C.c_int64()
         rc = hs.load().hs_cg_f64(dev.h, dA.p, drhs.p, dx.p, N, b, 1e-10, 100, C.byref(it), None, None)
         assert rc == hs.HS_ERR_WORKSPACE
         assert "hs_workspace_reserve" in hs.last_error()
-        # the factorisation's event pool was sized for n = 4 blocks as well
+        # the factorisation's event pool was sized for n = 1 block as well
         rc = hs.load().hs_cholesky_f64(dev.h, dA.p, N, b)
         assert rc == hs.HS_ERR_WORKSPACE
         assert dev.workspace_bytes() == w0
