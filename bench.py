@@ -151,12 +151,14 @@ def cholesky_workload(hs, dev, syn, args):
     }
 
 
-def mixed_workload(hs, dev, syn, args, x64=None):
-    """Mixed precision on the same N = 65536 system: time to solution of
-    (FP32-stored factor + FP64 refinement to backward error 1e-12), for the
-    FP32-MFMA and the BF16 hi+lo trailing updates, K assembled beforehand.
-    With the FP64 path's solution x64, each variant's relative forward
-    difference from it is reported as well."""
+def mixed_workload(hs, dev, syn, args, x64=None, rr64=None):
+    """Mixed precision on the same N = 65536 system: time to solution of an
+    FP32-stored factor (FP32-MFMA or BF16 hi+lo trailing updates) plus FP64
+    refinement, K assembled beforehand, to two targets: the normwise backward
+    error 1e-12, and the FP64 path's own relative residual rr64 (equal
+    residual) together with that backward error. A refinement that misses its
+    target fails the run. With the FP64 path's solution x64, each variant's
+    relative forward difference from it is reported."""
     seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
     if args.N:
         N = args.N
@@ -172,6 +174,9 @@ def mixed_workload(hs, dev, syn, args, x64=None):
     t_fac, t_ref = hs.Timer(dev), hs.Timer(dev)
     phase = (C.c_double * 3)()
     launches = (C.c_int64 * 3)()
+    targets = [("backward_error_1e-12", 1e-12, 0.0)]
+    if rr64:
+        targets.append(("equal_residual", 1e-12, rr64))
     out = {}
     for name, variant in (("fp32", hs.HS_MIXED_FP32), ("bf16x3", hs.HS_MIXED_BF16X3)):
         runs = []
@@ -181,28 +186,35 @@ def mixed_workload(hs, dev, syn, args, x64=None):
             t_fac.stop()
             if info != 0:
                 raise RuntimeError("mixed factorisation reported non-positive pivot %d" % info)
-            rep = hs.RefineReport()
-            t_ref.start()
-            rc = hs.load().hs_refine_solve(dev.h, dK.p, dL.p, dy.p, dx.p, N, b, 1e-12, 30, 300, C.byref(rep))
-            t_ref.stop()
-            if rc not in (0, hs.HS_ERR_NOT_CONVERGED):
-                raise hs.HsError(rc, "hs_refine_solve")
-            fac, ref = t_fac.ms(), t_ref.ms()
-            log("  mixed %s step %d: factor %.1f ms refine %.1f ms (%s)" % (name, s, fac, ref, rep.as_dict()))
+            fac = t_fac.ms()
+            per_target = {}
+            for tname, tol, rrtol in targets:
+                rep = hs.RefineReport()
+                t_ref.start()
+                hs.call("hs_refine_solve_to", dev.h, dK.p, dL.p, dy.p, dx.p, N, b, tol, rrtol, 60, 600,
+                        C.byref(rep))
+                t_ref.stop()
+                per_target[tname] = (t_ref.ms(), rep.as_dict(), dev.download(dx, n * b)[:N] if x64 is not None
+                                     else None)
+            log("  mixed %s step %d: factor %.1f ms; refine %s" % (
+                name, s, fac, {k: "%.1f ms (%s)" % (v[0], v[1]) for k, v in per_target.items()}))
             if s >= args.warmup:
-                runs.append((fac, ref, list(phase), rep.as_dict()))
+                runs.append((fac, list(phase), per_target))
         fac = float(np.median([r[0] for r in runs]))
-        ref = float(np.median([r[1] for r in runs]))
-        ph = np.mean([r[2] for r in runs], axis=0)
+        ph = np.mean([r[1] for r in runs], axis=0)
         _, upd_flops = mixed_update_work(hs, N, b, variant)
-        fwd = None
-        if x64 is not None:
-            xm = dev.download(dx, n * b)[:N]
-            fwd = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
-        out[name] = {"factor_ms": fac, "refine_ms": ref, "solve_ms": fac + ref, "report": runs[-1][3],
-                     "forward_diff_vs_fp64": fwd,
-                     "update_kernel_ms": float(ph[2]), "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
-                     "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}
+        rec = {"factor_ms": fac, "update_kernel_ms": float(ph[2]),
+               "update_tflops": upd_flops / (ph[2] / 1e3) / 1e12,
+               "kernel_sum_ms": {"potrf": float(ph[0]), "trsm": float(ph[1]), "update": float(ph[2])}}
+        for tname, tol, rrtol in targets:
+            ref = float(np.median([r[2][tname][0] for r in runs]))
+            last = runs[-1][2][tname]
+            fwd = None
+            if x64 is not None:
+                fwd = float(np.linalg.norm(last[2] - x64) / np.linalg.norm(x64))
+            rec[tname] = {"target_backward_error": tol, "target_relative_residual": rrtol or None,
+                          "refine_ms": ref, "solve_ms": fac + ref, "report": last[1], "forward_diff_vs_fp64": fwd}
+        out[name] = rec
     dK.free()
     dL.free()
     return out
@@ -378,30 +390,73 @@ def gpr_workload(hs, dev, syn, args):
 
 
 def cpu_baseline(args):
-    """The oracle (C restatement of the reference's CPU path, OpenMP) on a
-    bounded sample of the same workload: the N = 14336 system of the same
-    construction (d = 16, b = 256, l = 2, noise 1e-3), about 12 s of work on
-    16 host threads."""
+    """The oracle (C/OpenMP restatement of the reference's share-0 CPU path)
+    on the host cores, one entry per workload:
+      * cholesky_4096: the CPU-reference-path workload exactly (N = 4096,
+        b = 128, d = 8, l = 1, sn2 = 1e-2, seed 1): assembly + blocked
+        Cholesky + 2 triangular solves, whole;
+      * cholesky_65536_sample: the N = 65536 construction (d = 16, b = 256,
+        l = 2, sn2 = 1e-3, seed 3) at N = --cpu-n (14336 by default), the
+        same steps, in TFLOP/s (N^3/3 over the time);
+      * cg_32768_sample: the CG workload's system (N = 32768, b = 128,
+        seed 2), assembled on the host, then --cpu-cg-iters CG iterations of
+        the reference's CPU CG (matrixVectorBlock_CPU, chunked dot products),
+        in GB/s on the GPU figure's basis (packed bytes per mat-vec: the
+        iterations' mat-vecs, the %50 residual and the initial one).
+    The headline `value` is the 65536-construction sample, in the bench
+    line's unit."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from hs_testlib import Oracle
 
     syn = _load("hs_synth", os.path.join(PKG, "synthetic.py"))
     o = Oracle()
+    cores = int(o.or_num_threads())
+    per = {}
+
+    def chol(seed, N, d, b, ell, sf2, sn2):
+        X, y = syn.gp_inputs(seed, N, d)
+        t0 = time.time()
+        A = o.rbf(X, N, d, b, ell, sf2, sn2)
+        L = o.cholesky(A, N, b)
+        n = -(-N // b)
+        rhs = np.zeros(n * b)
+        rhs[:N] = y
+        o.tri_solve(L, rhs, N, b)
+        return time.time() - t0
+
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky_small"]
+    dt = chol(seed, N, d, b, ell, sf2, sn2)
+    per["cholesky_4096"] = {"value": N ** 3 / 3.0 / dt / 1e12, "unit": "TFLOP/s", "seconds": dt,
+                            "sample": "whole workload: assembly + Cholesky + 2 solves, N=%d b=%d d=%d" % (N, b, d)}
+    log("cpu baseline: N=%d Cholesky workload %.2f s" % (N, dt))
     seed, _, d, b, ell, sf2, sn2 = syn.WORKLOADS["cholesky"]
     N = args.cpu_n
+    dt = chol(seed, N, d, b, ell, sf2, sn2)
+    per["cholesky_65536_sample"] = {
+        "value": N ** 3 / 3.0 / dt / 1e12, "unit": "TFLOP/s", "seconds": dt,
+        "sample": "assembly + Cholesky + 2 solves of the N=65536 construction at N=%d, b=%d, d=%d" % (N, b, d)}
+    log("cpu baseline: N=%d sample of the 65536 construction %.2f s" % (N, dt))
+    seed, N, d, b, ell, sf2, sn2 = syn.WORKLOADS["cg"]
     X, y = syn.gp_inputs(seed, N, d)
-    t0 = time.time()
     A = o.rbf(X, N, d, b, ell, sf2, sn2)
-    L = o.cholesky(A, N, b)
     n = -(-N // b)
     rhs = np.zeros(n * b)
     rhs[:N] = y
-    o.tri_solve(L, rhs, N, b)
+    iters = args.cpu_cg_iters
+    t0 = time.time()
+    _, it, _ = o.cg(A, rhs, N, b, 1e-300, imax=iters, chunks=cores)
     dt = time.time() - t0
-    return {"value": N ** 3 / 3.0 / dt / 1e12, "unit": "TFLOP/s", "cores": int(o.or_num_threads()), "kind": "port",
-            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-            "sample": "oracle (C/OpenMP restatement of the reference CPU path): assembly + blocked Cholesky + "
-                      "2 triangular solves at N=%d, b=%d, d=%d, %.1f s" % (N, b, d, dt)}
+    nsymv = it + -(-it // 50) + 1
+    per["cg_32768_sample"] = {
+        "value": nsymv * packed_len(N, b) * 8 / dt / 1e9, "unit": "GB/s", "seconds": dt, "iterations": it,
+        "sample": "%d CG iterations (%d mat-vecs) of the N=%d, b=%d CG workload; %.2f s per iteration"
+                  % (it, nsymv, N, b, dt / it)}
+    log("cpu baseline: %d CG iterations at N=%d in %.2f s" % (it, N, dt))
+    del A
+    head = per["cholesky_65536_sample"]
+    return {"value": head["value"], "unit": "TFLOP/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(), "sample": "oracle (C/OpenMP restatement of the reference's CPU path): " +
+            head["sample"] + ", %.1f s" % head["seconds"], "per_workload": per}
 
 
 def cpu_model():
@@ -427,6 +482,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=14336)
+    ap.add_argument("--cpu-cg-iters", type=int, default=30)
     ap.add_argument("--no-gpr", action="store_true")
     ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr", "mixed"],
                     help="run only this side workload and print its record (for profiling; not a bench line)")
@@ -470,14 +526,16 @@ def main():
     with hs.Device(0, reserve=sizes) as dev:
         ch = cholesky_workload(hs, dev, syn, args)
         x64 = ch.pop("_x")
-        mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args, x64)
+        mixed = None if args.no_mixed else mixed_workload(hs, dev, syn, args, x64, ch["rel_residual"])
         cg = None if args.no_cg else cg_workload(hs, dev, syn, args)
         adj = None if args.no_adjoint else adjoint_workload(hs, dev, syn, args)
         gpr = None if args.no_gpr else gpr_workload(hs, dev, syn, args)
     if mixed:
         fp64_solve = ch["factor_ms"] + ch["solve_ms"]
         for v in mixed.values():
-            v["speedup_vs_fp64_factor_plus_solve"] = fp64_solve / v["solve_ms"]
+            for t in ("backward_error_1e-12", "equal_residual"):
+                if t in v:
+                    v[t]["speedup_vs_fp64_factor_plus_solve"] = fp64_solve / v[t]["solve_ms"]
     cpu = None if args.no_cpu else cpu_baseline(args)
     up = ch["update"]
     out = {

@@ -150,22 +150,13 @@ size_t hs_ws_refine(i64 N, int b) {
     return sizeof(double) * (size_t)(hs_dinv_doubles((int)n, b) + 2 * nb * b + 5 * rows + G + 16);
 }
 
-extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x,
-                               int64_t N, int b, double tol, int max_ir, int max_pcg, hs_refine_report* rep) {
-    if (!h) return -1;
-    if (!K) return -2;
-    if (!L32) return -3;
-    if (!y) return -4;
-    if (!x) return -5;
-    if (N <= 0) return -6;
-    if (!hs_trsv_fast_ok(b)) {
-        hs_set_error("hs_refine_solve: matrix_bsz must be 64, 128 or 256");
-        return -7;
-    }
-    if (!(tol > 0.0)) return -8;
-    if (max_ir < 0) return -9;
-    if (max_pcg < 0) return -10;
-    if (!rep) return -11;
+// The refinement loop behind both entries. Targets: the normwise backward
+// error omega <= tol (skipped when tol == 0) and the relative residual
+// ||y - K x||_2 / ||y||_2 <= rrtol (skipped when rrtol == 0); both must hold.
+// Progress is measured as e = max(omega / tol, rr / rrtol) over the active
+// targets: a refinement step that does not halve e hands over to PCG.
+static int refine(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N, int b,
+                  double tol, double rrtol, int max_ir, int max_pcg, hs_refine_report* rep) {
     Ctx c;
     c.h = h;
     c.K = K;
@@ -205,31 +196,45 @@ extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32,
     *rep = hs_refine_report{0, 0, 0, 0, 0.0, 0.0, 0.0};
 
     if ((rc = hs_diag_inverse<float>(h, L32, N, b, c.Dinv))) return rc;
-    // ||K||_inf = max row sum of |K|, ||y||_inf
+    // ||K||_inf = max row sum of |K|, ||y||_inf, ||y||_2
     fill_ones_kernel<<<c.vg, c.vb, 0, h->stream>>>(c.d, c.rows, N);
     HS_LAUNCH_CHECK();
     if ((rc = hs_symv_into(h, K, c.d, c.q, nullptr, N, b, c.rowpart, c.colpart, c.part, 0, true))) return rc;
-    double knorm = 0.0, ynorm = 0.0;
-    if ((rc = c.maxabs2(c.q, y, &knorm, &ynorm))) return rc;
+    double knorm = 0.0, ynorm = 0.0, yy = 0.0;
+    if ((rc = c.maxabs2(c.q, y, &knorm, &ynorm)) || (rc = c.dot(y, y, &yy))) return rc;
+    const double ynorm2 = std::sqrt(yy);
     rep->knorm_inf = knorm;
+
+    // the state of the current residual r = y - K x: omega, rr, and the
+    // progress measure e (<= 1 when every active target holds)
+    double omega = 0.0, rr = 0.0;
+    auto measure = [&](double* e) -> int {
+        double rmax, xmax, r2 = 0.0;
+        int e1 = c.maxabs2(c.r, x, &rmax, &xmax);
+        if (e1) return e1;
+        omega = rmax / (knorm * xmax + ynorm);
+        if (rrtol > 0.0 && (e1 = c.dot(c.r, c.r, &r2))) return e1;
+        rr = ynorm2 > 0.0 ? std::sqrt(r2) / ynorm2 : 0.0;
+        *e = std::max(tol > 0.0 ? omega / tol : 0.0, rrtol > 0.0 ? rr / rrtol : 0.0);
+        return HS_OK;
+    };
 
     // x0 = (L L^T)^{-1} y
     if ((rc = c.copy(y, x)) || (rc = c.solve(x))) return rc;
-    double omega = 0.0, omegaPrev = 0.0, rmax, xmax;
+    double e = 0.0, ePrev = 0.0;
     bool converged = false, stalled = false;
     for (int it = 0;; ++it) {
-        if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
-        omega = rmax / (knorm * xmax + ynorm);
-        if (omega <= tol) {
+        if ((rc = c.residual(y, x)) || (rc = measure(&e))) return rc;
+        if (e <= 1.0) {
             converged = true;
             break;
         }
-        if (it >= 1 && omega > 0.5 * omegaPrev) {
+        if (it >= 1 && e > 0.5 * ePrev) {
             stalled = true;
             break;
         }
         if (it >= max_ir) break;
-        omegaPrev = omega;
+        ePrev = e;
         if ((rc = c.copy(c.r, c.d)) || (rc = c.solve(c.d)) || (rc = c.axpy(x, c.d, 1.0))) return rc;
         rep->ir_iterations = it + 1;
     }
@@ -245,12 +250,11 @@ extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32,
             const double alpha = rho / pq;
             if ((rc = c.axpy(x, c.p, alpha)) || (rc = c.axpy(c.r, c.q, -alpha))) return rc;
             rep->pcg_iterations = k;
-            if ((rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
-            if (rmax / (knorm * xmax + ynorm) <= tol) {
+            if ((rc = measure(&e))) return rc;
+            if (e <= 1.0) {
                 // confirm on the true residual before stopping
-                if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
-                omega = rmax / (knorm * xmax + ynorm);
-                if (omega <= tol) {
+                if ((rc = c.residual(y, x)) || (rc = measure(&e))) return rc;
+                if (e <= 1.0) {
                     converged = true;
                     break;
                 }
@@ -262,18 +266,53 @@ extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32,
             rho = rhoNew;
         }
         if (!converged) {
-            if ((rc = c.residual(y, x)) || (rc = c.maxabs2(c.r, x, &rmax, &xmax))) return rc;
-            omega = rmax / (knorm * xmax + ynorm);
+            if ((rc = c.residual(y, x)) || (rc = measure(&e))) return rc;
         }
     }
-    double rr = 0.0, yy = 0.0;
-    if ((rc = c.dot(c.r, c.r, &rr)) || (rc = c.dot(y, y, &yy))) return rc;
+    double r2 = 0.0;
+    if ((rc = c.dot(c.r, c.r, &r2))) return rc;
     rep->backward_error = omega;
-    rep->relative_residual = std::sqrt(rr / yy);
-    rep->converged = omega <= tol ? 1 : 0;
+    rep->relative_residual = ynorm2 > 0.0 ? std::sqrt(r2) / ynorm2 : 0.0;
+    rep->converged = e <= 1.0 ? 1 : 0;
     if (!rep->converged) {
-        hs_set_error("hs_refine_solve: backward error target not reached");
+        hs_set_error("hs_refine_solve: target not reached (backward error " + std::to_string(omega) +
+                     ", relative residual " + std::to_string(rep->relative_residual) + ")");
         return HS_ERR_NOT_CONVERGED;
     }
     return HS_OK;
+}
+
+static int refine_args(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N,
+                       int b, int max_ir, int max_pcg, hs_refine_report* rep) {
+    if (!h) return -1;
+    if (!K) return -2;
+    if (!L32) return -3;
+    if (!y) return -4;
+    if (!x) return -5;
+    if (N <= 0) return -6;
+    if (!hs_trsv_fast_ok(b)) {
+        hs_set_error("hs_refine_solve: matrix_bsz must be 64, 128 or 256");
+        return -7;
+    }
+    if (max_ir < 0) return -9;
+    if (max_pcg < 0) return -10;
+    if (!rep) return -11;
+    return HS_OK;
+}
+
+extern "C" int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x,
+                               int64_t N, int b, double tol, int max_ir, int max_pcg, hs_refine_report* rep) {
+    int rc = refine_args(h, K, L32, y, x, N, b, max_ir, max_pcg, rep);
+    if (rc) return rc;
+    if (!(tol > 0.0)) return -8;
+    return refine(h, K, L32, y, x, N, b, tol, 0.0, max_ir, max_pcg, rep);
+}
+
+extern "C" int hs_refine_solve_to(hs_handle_t h, const double* K, const float* L32, const double* y, double* x,
+                                  int64_t N, int b, double tol, double rel_res_tol, int max_ir, int max_pcg,
+                                  hs_refine_report* rep) {
+    int rc = refine_args(h, K, L32, y, x, N, b, max_ir, max_pcg, rep);
+    if (rc) return rc;
+    if (!(tol >= 0.0) || !(rel_res_tol >= 0.0) || (tol == 0.0 && rel_res_tol == 0.0)) return -8;
+    return refine(h, K, L32, y, x, N, b, tol, rel_res_tol, max_ir, max_pcg, rep);
 }

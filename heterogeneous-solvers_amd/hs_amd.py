@@ -83,6 +83,7 @@ _SIGS = {
     "hs_cholesky_mixed": [_P, _P, _P, _L, _I, _I, _D],
     "hs_cholesky_mixed_timed": [_P, _P, _P, _L, _I, _I, _D, C.POINTER(_D), C.POINTER(_L)],
     "hs_refine_solve": [_P, _P, _P, _P, _P, _L, _I, _D, _I, _I, _P],
+    "hs_refine_solve_to": [_P, _P, _P, _P, _P, _L, _I, _D, _D, _I, _I, _P],
     "hs_trsv_packed_f64": [_P, _P, _P, _L, _I, _I],
     "hs_gp_nll_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P, _P, C.POINTER(_D)],
     "hs_gp_nll_grad_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P, _P, _P, C.POINTER(_D), C.POINTER(_D)],

@@ -291,6 +291,12 @@ typedef struct hs_refine_report_s {
  * HS_ERR_NOT_CONVERGED with the report filled in. */
 int hs_refine_solve(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N, int b,
                     double tol, int max_ir, int max_pcg, hs_refine_report* report);
+/* hs_refine_solve with a second target: the relative residual
+ * ||y - K x||_2 / ||y||_2 <= rel_res_tol as well as the backward error
+ * <= tol (either one switched off by 0, not both), so that a mixed-precision
+ * solve can be run to the residual an FP64 solve reached. */
+int hs_refine_solve_to(hs_handle_t h, const double* K, const float* L32, const double* y, double* x, int64_t N, int b,
+                       double tol, double rel_res_tol, int max_ir, int max_pcg, hs_refine_report* report);
 
 /* ---- Gaussian-process likelihood (no reference counterpart; the forward
  * pass is the reference's kernel matrix, MatrixGenerator.cpp:143-153, and its

@@ -1,8 +1,9 @@
 """GP NLL gradient by the hand-written reverse pass (hs_gp_nll_grad_f64):
 against the FP64 closed form with an explicit K^{-1} (committed fixtures,
 tests/golden/make_gp_nll_fixtures.py), against dense numpy dNLL/dK, against
-central finite differences (h = 1e-5 in log space) at the 16384-point
-workload, and through torch.autograd (gradcheck)."""
+central finite differences (h = 1e-5 in log space) and an FP64 CPU reference
+gradient at the 16384-point workload, and through torch.autograd
+(gradcheck)."""
 import json
 import math
 import os
@@ -32,14 +33,24 @@ def test_nll_grad_matches_closed_form(hs, dev, fx):
     X, y = _inputs(fx["seed"], fx["N"], fx["d"])
     theta = [math.log(fx["ell"]), math.log(fx["sf2"]), math.log(fx["sn2"])]
     nll, g, Kb = ga.nll_grad(theta, X, y, fx["b"], want_kbar=True)
-    print("  nll %.15g (ref %.15g)  grad %s (ref %s)" % (nll, fx["nll"], g, fx["grad"]))
-    # FP64 agreement is limited by cond(K) * eps on K^{-1}
-    tol = max(1e-11, 4e-16 * fx["cond"])
-    assert abs(nll - fx["nll"]) <= tol * max(1.0, abs(fx["nll"]))
-    # the gradient is a sum of N^2 terms of size up to |Kbar|: its error scales
-    # with sum |Kbar|, not with the (cancelled) result
+    print("  nll %.15g (ref %.15g)  grad %s (ref %s)" % (nll, fx["nll"], g, fx["grad"]), flush=True)
     gref = np.array(fx["grad"])
-    assert np.max(np.abs(np.array(g) - gref)) <= 1e-12 * fx["kbar_abs_sum"] + 1e-9 * np.max(np.abs(gref))
+    if fx["cond"] is None:
+        # the gradient workload (N = 16384): the reference's own spread (two
+        # independent inverses) is ~1e-14, so the bar is the 1e-9 relative
+        # agreement of an FP64 reverse pass with an FP64 CPU reference
+        rel = np.max(np.abs(np.array(g) - gref)) / np.max(np.abs(gref))
+        print("  max relative gradient difference %.3e (reference spread %.1e)" % (rel, fx["ref_spread"]))
+        assert rel <= 1e-9
+        assert abs(nll - fx["nll"]) <= 1e-10 * abs(fx["nll"])
+        tol = 1e-10
+    else:
+        # FP64 agreement is limited by cond(K) * eps on K^{-1}
+        tol = max(1e-11, 4e-16 * fx["cond"])
+        assert abs(nll - fx["nll"]) <= tol * max(1.0, abs(fx["nll"]))
+        # the gradient is a sum of N^2 terms of size up to |Kbar|: its error
+        # scales with sum |Kbar|, not with the (cancelled) result
+        assert np.max(np.abs(np.array(g) - gref)) <= 1e-12 * fx["kbar_abs_sum"] + 1e-9 * np.max(np.abs(gref))
     low = packed_to_dense(Kb.cpu().numpy(), fx["N"], fx["b"], lower_only=True)
     assert abs(low.sum() - fx["kbar_sum"]) <= tol * fx["kbar_abs_sum"]
     assert abs(np.abs(low).sum() - fx["kbar_abs_sum"]) <= tol * fx["kbar_abs_sum"]

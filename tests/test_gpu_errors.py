@@ -54,3 +54,6 @@ def test_refine_argument_codes(hs, dev):
     assert lib.hs_refine_solve(dev.h, p, p, p, p, 128, 96, 1e-12, 5, 5, C.byref(rep)) == -7
     assert lib.hs_refine_solve(dev.h, p, p, p, p, 128, 128, 0.0, 5, 5, C.byref(rep)) == -8
     assert lib.hs_refine_solve(dev.h, p, p, p, p, 128, 128, 1e-12, -1, 5, C.byref(rep)) == -9
+    # the relative-residual form: at least one target, none negative
+    assert lib.hs_refine_solve_to(dev.h, p, p, p, p, 128, 128, 0.0, 0.0, 5, 5, C.byref(rep)) == -8
+    assert lib.hs_refine_solve_to(dev.h, p, p, p, p, 128, 128, 1e-12, -1.0, 5, 5, C.byref(rep)) == -8

@@ -85,6 +85,37 @@ def test_refine_reaches_fp64_backward_error(hs, dev, oracle, variant, N, b, ell,
     assert np.all(x[N:] == 0.0)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N,b,ell,sn2", [(2048, 256, 2.0, 1e-3), (3000, 128, 1.0, 1e-2)])
+def test_refine_to_fp64_relative_residual(hs, dev, oracle, variant, N, b, ell, sn2):
+    """Equal residual: refinement run to the relative residual the FP64
+    factor + solves reach on the same system (and the 1e-12 backward error),
+    checked on the host against the dense K."""
+    print("refine to equal residual variant=%d N=%d b=%d" % (variant, N, b), flush=True)
+    A, rhs = system(oracle, N, 16, b, ell, sn2, 9)
+    K = packed_to_dense(A, N, b)
+    # the FP64 path on the GPU: factor + solves, its relative residual
+    dA, dx64 = dev.upload(A), dev.upload(rhs)
+    assert hs.call("hs_cholesky_f64", dev.h, dA.p, N, b) == 0
+    hs.call("hs_tri_solve_f64", dev.h, dA.p, dx64.p, N, b)
+    x64 = dev.download(dx64, len(rhs))
+    rr64 = np.linalg.norm(rhs[:N] - K @ x64[:N]) / np.linalg.norm(rhs[:N])
+    dK, dL, dy, dx = dev.upload(A), dev.alloc(packed_len(N, b) * 4), dev.upload(rhs), dev.zeros(len(rhs))
+    assert hs.call("hs_cholesky_mixed", dev.h, dK.p, dL.p, N, b, variant, 0.0) == 0
+    rep = hs.RefineReport()
+    hs.call("hs_refine_solve_to", dev.h, dK.p, dL.p, dy.p, dx.p, N, b, 1e-12, rr64, 60, 600, C.byref(rep))
+    x = dev.download(dx, len(rhs))
+    rr = np.linalg.norm(rhs[:N] - K @ x[:N]) / np.linalg.norm(rhs[:N])
+    fwd = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    print("  FP64 relative residual %.3e; mixed %.3e (device %.3e); report %s; forward diff vs FP64 %.2e"
+          % (rr64, rr, rep.relative_residual, rep.as_dict(), fwd), flush=True)
+    assert rep.converged == 1
+    assert rep.relative_residual <= rr64
+    # the host recomputation (different summation order) within rounding of it
+    assert rr <= 2 * rr64
+    assert np.all(x[N:] == 0.0)
+
+
 def test_pcg_fallback_and_shift(hs, dev, oracle):
     """A shifted factor makes refinement contract slowly; the solve must switch
     to PCG and still reach the target. max_ir = 0 goes to PCG at once."""
