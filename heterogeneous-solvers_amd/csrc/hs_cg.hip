@@ -14,7 +14,9 @@
 // until an LDS combine.
 // The result is bit-reproducible run to run (no atomics).
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <mutex>
 
 #include "hs_internal.h"
 
@@ -493,26 +495,31 @@ extern "C" int hs_symv_packed_f64_timed(hs_handle_t h, const double* A, const do
 // chunks; each iteration's kernels check the device stop index first, so
 // iterations enqueued past convergence cost only their launch. The stop
 // index is read back once per chunk through pinned memory.
-extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
-                         int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero) {
+static int cg_args(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                   int64_t imax, const char* who) {
     if (!h) return -1;
     if (!A) return -2;
     if (!rhs) return -3;
     if (!x) return -4;
     if (N <= 0) return -5;
     if (!symv_fast_ok(b)) {
-        hs_set_error("hs_cg_f64: block size must be 64, 128 or 256");
+        hs_set_error(std::string(who) + ": block size must be 64, 128 or 256");
         return -6;
     }
     if (!(eps >= 0.0)) return -7;
     if (imax < 0) return -8;
+    return HS_OK;
+}
+
+// The device-resident CG on buffers carved out of `w` (symv_carve with
+// vectors): no allocation, the handle's pre-created events and pinned word.
+static int cg_run(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                  int64_t imax, const SymvWork& w, int64_t* iterations, double* delta_final, double* delta_zero) {
     const int n = (int)((N + b - 1) / b);
     const i64 rows = (i64)n * b;
     const int G = (int)((rows + SYMV_THREADS - 1) / SYMV_THREADS);  // vector-kernel grid
     const int GF = (int)(rows / FIN_ROWS);                          // reduction-pass grid
-    SymvWork w;
-    int rc = symv_workspace(h, n, b, true, &w);
-    if (rc) return rc;
+    int rc = HS_OK;
     double* d = w.vec;
     double* q = w.vec + rows;
     double* r = w.vec + 2 * rows;
@@ -579,14 +586,86 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
         slot ^= 1;
     }
     if (rc) return rc;
+    // the final state through the handle's pinned page (past the two stop slots)
+    CGState* pinnedState = (CGState*)((char*)h->pinned + 64);
+    HS_HIP(hipMemcpyAsync(pinnedState, st, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
     HS_HIP(hipStreamSynchronize(h->stream));
-    CGState hs;
-    HS_HIP(hipMemcpy(&hs, st, sizeof(CGState), hipMemcpyDeviceToHost));
+    const CGState hs = *pinnedState;
     const i64 iters = (hs.stop == LLONG_MAX) ? enq : hs.stop + 1;
     if (iterations) *iterations = iters;
     if (delta_final) *delta_final = hs.delta[iters & 1];
     if (delta_zero) *delta_zero = hs.delta0;
     return HS_OK;
+}
+
+extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                         int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero) {
+    int rc = cg_args(h, A, rhs, x, N, b, eps, imax, "hs_cg_f64");
+    if (rc) return rc;
+    SymvWork w;
+    if ((rc = symv_workspace(h, (int)((N + b - 1) / b), b, true, &w))) return rc;
+    return cg_run(h, A, rhs, x, N, b, eps, imax, w, iterations, delta_final, delta_zero);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free CG (hs_cg_f64_noalloc) for callers that cannot allocate
+// device memory (graph capture, serving loops). Its scratch, the SYMV
+// partials, the three CG vectors and the loop state, lives in `scratch`, a
+// file-scope device array of HS_NOALLOC_SCRATCH_BYTES that every GPU the
+// library runs on holds until the process ends. On the host the symbol names
+// a stand-in, not device memory: hs_noalloc_setup looks up its device address
+// once per device (from hs_handle_create, with that device current) and only
+// that address is ever handed to kernels. Every call on one GPU shares the
+// array: calls that overlap on one GPU race. Nothing in it has to start at
+// zero: each call writes every partial and vector it later reads, and its
+// init kernels set the loop state.
+constexpr size_t HS_NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+constexpr int HS_NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[HS_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::atomic<void*> g_scratch_dev[HS_NOALLOC_MAX_DEVICES];
+
+int hs_noalloc_setup(int device) {
+    if (device < 0 || device >= HS_NOALLOC_MAX_DEVICES) {
+        hs_set_error("hs_noalloc_setup: device index beyond the supported range");
+        return HS_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device].load()) return HS_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hs_hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[device].store(p);
+    return HS_OK;
+}
+
+extern "C" int hs_cg_noalloc_limit(size_t* scratch_bytes) {
+    if (!scratch_bytes) return -1;
+    *scratch_bytes = HS_NOALLOC_SCRATCH_BYTES;
+    return HS_OK;
+}
+
+extern "C" int hs_cg_f64_noalloc(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b,
+                                 double eps, int64_t imax, int64_t* iterations, double* delta_final,
+                                 double* delta_zero) {
+    int rc = cg_args(h, A, rhs, x, N, b, eps, imax, "hs_cg_f64_noalloc");
+    if (rc) return rc;
+    const size_t need = symv_work_bytes(N, b, true);
+    if (need > HS_NOALLOC_SCRATCH_BYTES) {
+        hs_set_error("hs_cg_f64_noalloc: N = " + std::to_string(N) + ", b = " + std::to_string(b) + " needs " +
+                     std::to_string(need) + " bytes of scratch, beyond its fixed " +
+                     std::to_string(HS_NOALLOC_SCRATCH_BYTES) + " (use hs_cg_f64 with a reserved workspace)");
+        return HS_ERR_UNSUPPORTED;
+    }
+    void* base = (h->device >= 0 && h->device < HS_NOALLOC_MAX_DEVICES) ? g_scratch_dev[h->device].load() : nullptr;
+    if (!base) {
+        hs_set_error("hs_cg_f64_noalloc: no scratch address for this device (handle not created by hs_handle_create)");
+        return HS_ERR_UNSUPPORTED;
+    }
+    SymvWork w;
+    symv_carve(base, (int)((N + b - 1) / b), b, true, &w);
+    return cg_run(h, A, rhs, x, N, b, eps, imax, w, iterations, delta_final, delta_zero);
 }
 
 // Internal SYMV for other solvers: mode 0 y = A x, mode 2 y = rhs - A x (part

@@ -57,6 +57,8 @@ size_t hs_ws_mixed(i64 N, int b);
 size_t hs_ws_adjoint(i64 N, int b);
 size_t hs_ws_gp_predict(i64 N, i64 M);
 size_t hs_ws_rbf_window(i64 N);
+// Looks up the allocation-free CG's scratch address for `device` (current).
+int hs_noalloc_setup(int device);
 // Ensures the handle's event pool holds at least `count` events.
 int hs_ensure_events(hs_handle_t h, size_t count);
 

@@ -53,6 +53,8 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e == hipSuccess) h->events.push_back(ev);
     }
+    // the device address of the allocation-free CG's scratch, once per device
+    if (e == hipSuccess && hs_noalloc_setup(device) != HS_OK) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         for (auto ev : h->events) hipEventDestroy(ev);
         hipStreamDestroy(h->stream2);

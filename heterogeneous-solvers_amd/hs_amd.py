@@ -73,6 +73,8 @@ _SIGS = {
     "hs_cholesky_mixed_schedule": [_L, _I, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_read_sum_f64": [_P, _P, _L, _P],
     "hs_cg_f64": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],
+    "hs_cg_f64_noalloc": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],
+    "hs_cg_noalloc_limit": [C.POINTER(C.c_size_t)],
     "hs_cg_ref_order_f64": [_P, _P, _P, _P, _L, _I, _D, _L, _I, _I, _I, C.POINTER(_L), C.POINTER(_D),
                             C.POINTER(_D)],
     "hs_symv_ref_order_f64": [_P, _P, _P, _P, _L, _I, _I],

@@ -219,6 +219,18 @@ int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int64_t N, int
 int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
               int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero);
 
+/* hs_cg_f64 without any allocation and without the handle workspace: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (graph capture, serving loops). Its scratch (SYMV partials, CG
+ * vectors, loop state) is a fixed device array of the library,
+ * hs_cg_noalloc_limit bytes (32 MiB: N up to about 22000 at b = 128),
+ * held by every GPU the library runs on until the process ends; larger
+ * systems return HS_ERR_UNSUPPORTED. Calls that overlap on one GPU share that
+ * array and race. Same kernels and results as hs_cg_f64. */
+int hs_cg_f64_noalloc(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                      int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero);
+int hs_cg_noalloc_limit(size_t* scratch_bytes);
+
 /* The same CG in the reference's own floating-point order, the parity mode:
  * the mat-vec sums every row over block columns 0..n-1 and columns 0..b-1 in
  * order (matrixVectorBlock_GPU, MatrixVectorOperations.cpp:95-192), the dot
