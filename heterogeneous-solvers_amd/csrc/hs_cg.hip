@@ -514,7 +514,8 @@ static int cg_args(hs_handle_t h, const double* A, const double* rhs, double* x,
 // The device-resident CG on buffers carved out of `w` (symv_carve with
 // vectors): no allocation, the handle's pre-created events and pinned word.
 static int cg_run(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
-                  int64_t imax, const SymvWork& w, int64_t* iterations, double* delta_final, double* delta_zero) {
+                  int64_t imax, const SymvWork& w, int64_t* iterations, double* delta_final, double* delta_zero,
+                  double* trace = nullptr) {
     const int n = (int)((N + b - 1) / b);
     const i64 rows = (i64)n * b;
     const int G = (int)((rows + SYMV_THREADS - 1) / SYMV_THREADS);  // vector-kernel grid
@@ -538,17 +539,23 @@ static int cg_run(hs_handle_t h, const double* A, const double* rhs, double* x, 
     i64 stopHost = LLONG_MAX;
     hipEvent_t evs[2] = {h->events[0], h->events[1]};
     int pending = -1;  // slot of the chunk whose stop copy is in flight
+    IterTrace tr;
+    if ((rc = tr.start(trace, imax, CHUNK))) return rc;
+    i64 chunkFirst[2] = {0, 0}, chunkEnd[2] = {0, 0};  // iterations of the chunk in each slot
     auto enqueue_chunk = [&]() -> int {
         const i64 end = std::min<i64>(enq + CHUNK, imax);
         for (; enq < end; ++enq) {
             const i64 it = enq;
             int e;
+            if ((e = tr.mark(h->stream, it, 0))) return e;
             if ((e = launch_partials(h, A, d, w.rowpart, w.colpart, n, b, stop, it))) return e;
             symv_finish_kernel<1><<<fg, vb, 0, h->stream>>>(w.rowpart, w.colpart, n, b, q, d, w.part1, stop, it);
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 1))) return e;
             const int updateR = (it % 50) != 0;
             cg_xr_kernel<<<vg, vb, 0, h->stream>>>(st, w.part1, GF, d, q, x, r, rows, w.part2, it, updateR);
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 2))) return e;
             if (!updateR) {
                 // true residual every 50 iterations (CG.cpp:95-101)
                 if ((e = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, stop, it))) return e;
@@ -556,22 +563,32 @@ static int cg_run(hs_handle_t h, const double* A, const double* rhs, double* x, 
                                                                 it);
                 HS_LAUNCH_CHECK();
             }
+            if ((e = tr.mark(h->stream, it, 3))) return e;
             // r.r partials come from cg_xr (G of them) or from the residual pass (GF)
             cg_d_kernel<<<vg, vb, 0, h->stream>>>(st, w.part2, updateR ? G : GF, r, d, rows, it);
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 4))) return e;
         }
         return HS_OK;
+    };
+    // the executed iterations of the chunk in slot s, once it has completed
+    auto read_slot = [&](int s, i64 stopIt) -> int {
+        const i64 last = stopIt == LLONG_MAX ? chunkEnd[s] : std::min<i64>(chunkEnd[s], stopIt + 1);
+        return tr.read(chunkFirst[s], last);
     };
     int slot = 0;
     while (true) {
         if (enq < imax) {
+            chunkFirst[slot] = enq;
             if ((rc = enqueue_chunk())) break;
+            chunkEnd[slot] = enq;
             HS_HIP(hipMemcpyAsync((void*)(pinnedStop + slot), stop, sizeof(i64), hipMemcpyDeviceToHost, h->stream));
             HS_HIP(hipEventRecord(evs[slot], h->stream));
         }
         if (pending >= 0) {
             HS_HIP(hipEventSynchronize(evs[pending]));
             stopHost = pinnedStop[pending];
+            if ((rc = read_slot(pending, stopHost))) break;
             if (stopHost != LLONG_MAX) break;
         }
         if (enq >= imax) {
@@ -579,6 +596,7 @@ static int cg_run(hs_handle_t h, const double* A, const double* rhs, double* x, 
             if (pending != slot) {
                 HS_HIP(hipEventSynchronize(evs[slot]));
                 stopHost = pinnedStop[slot];
+                if ((rc = read_slot(slot, stopHost))) break;
             }
             break;
         }
@@ -605,6 +623,17 @@ extern "C" int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, doub
     SymvWork w;
     if ((rc = symv_workspace(h, (int)((N + b - 1) / b), b, true, &w))) return rc;
     return cg_run(h, A, rhs, x, N, b, eps, imax, w, iterations, delta_final, delta_zero);
+}
+
+extern "C" int hs_cg_f64_trace(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b,
+                               double eps, int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero,
+                               double* trace) {
+    int rc = cg_args(h, A, rhs, x, N, b, eps, imax, "hs_cg_f64_trace");
+    if (rc) return rc;
+    if (!trace) return -12;
+    SymvWork w;
+    if ((rc = symv_workspace(h, (int)((N + b - 1) / b), b, true, &w))) return rc;
+    return cg_run(h, A, rhs, x, N, b, eps, imax, w, iterations, delta_final, delta_zero, trace);
 }
 
 // ---------------------------------------------------------------------------

@@ -330,9 +330,9 @@ extern "C" int hs_dot_ref_order_f64(hs_handle_t h, const double* x, const double
     return launch_dot_tree(h, x, y, (i64)blockStart * b, len, wg, wgf, fma, part, result_dev, nullptr, 0);
 }
 
-extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b,
-                                   double eps, int64_t imax, int wg, int wgf, int fma, int64_t* iterations,
-                                   double* delta_final, double* delta_zero) {
+static int cg_ref_run(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                      int64_t imax, int wg, int wgf, int fma, int64_t* iterations, double* delta_final,
+                      double* delta_zero, double* trace) {
     if (!h) return -1;
     if (!A) return -2;
     if (!rhs) return -3;
@@ -379,19 +379,25 @@ extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double*
     volatile i64* pinnedStop = (volatile i64*)h->pinned;
     i64 enq = 0;
     hipEvent_t evs[2] = {h->events[0], h->events[1]};
+    IterTrace tr;
+    if ((rc = tr.start(trace, imax, CHUNK))) return rc;
+    i64 chunkFirst[2] = {0, 0}, chunkEnd[2] = {0, 0};
     auto enqueue_chunk = [&]() -> int {
         const i64 end = std::min<i64>(enq + CHUNK, imax);
         for (; enq < end; ++enq) {
             const i64 it = enq;
             int e;
             // q = A d ; alpha = delta_new / d.q ; x = x + alpha d
+            if ((e = tr.mark(h->stream, it, 0))) return e;
             if ((e = hs_launch_symv_ordered(h, A, d, q, (int)n, b, fma, stop, it))) return e;
+            if ((e = tr.mark(h->stream, it, 1))) return e;
             if ((e = launch_dot_tree(h, d, q, 0, rows, wg, wgf, fma, part, &st->dq, stop, it))) return e;
             if (fma)
                 ref_update_x_kernel<true><<<vg, vb, 0, h->stream>>>(st, d, x, rows, it);
             else
                 ref_update_x_kernel<false><<<vg, vb, 0, h->stream>>>(st, d, x, rows, it);
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 2))) return e;
             if (it % 50 == 0) {
                 // true residual (CG.cpp:95-101, 444-473)
                 if ((e = hs_launch_symv_ordered(h, A, x, r, (int)n, b, fma, stop, it))) return e;
@@ -402,6 +408,7 @@ extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double*
                 ref_update_r_kernel<false><<<vg, vb, 0, h->stream>>>(st, q, r, rows, it);
             }
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 3))) return e;
             // delta_new = r.r ; beta ; d = r + beta d ; stop test
             if ((e = launch_dot_tree(h, r, r, 0, rows, wg, wgf, fma, part, &st->delta[(it + 1) & 1], stop, it)))
                 return e;
@@ -410,22 +417,34 @@ extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double*
             else
                 ref_update_d_kernel<false><<<vg, vb, 0, h->stream>>>(st, r, d, rows, it);
             HS_LAUNCH_CHECK();
+            if ((e = tr.mark(h->stream, it, 4))) return e;
         }
         return HS_OK;
+    };
+    auto read_slot = [&](int s, i64 stopIt) -> int {
+        const i64 last = stopIt == LLONG_MAX ? chunkEnd[s] : std::min<i64>(chunkEnd[s], stopIt + 1);
+        return tr.read(chunkFirst[s], last);
     };
     int slot = 0, pending = -1;
     while (true) {
         if (enq < imax) {
+            chunkFirst[slot] = enq;
             if ((rc = enqueue_chunk())) break;
+            chunkEnd[slot] = enq;
             HS_HIP(hipMemcpyAsync((void*)(pinnedStop + slot), stop, sizeof(i64), hipMemcpyDeviceToHost, h->stream));
             HS_HIP(hipEventRecord(evs[slot], h->stream));
         }
         if (pending >= 0) {
             HS_HIP(hipEventSynchronize(evs[pending]));
-            if (pinnedStop[pending] != LLONG_MAX) break;
+            const i64 stopHost = pinnedStop[pending];
+            if ((rc = read_slot(pending, stopHost))) break;
+            if (stopHost != LLONG_MAX) break;
         }
         if (enq >= imax) {
-            if (pending != slot) HS_HIP(hipEventSynchronize(evs[slot]));
+            if (pending != slot) {
+                HS_HIP(hipEventSynchronize(evs[slot]));
+                if ((rc = read_slot(slot, pinnedStop[slot]))) break;
+            }
             break;
         }
         pending = slot;
@@ -440,4 +459,18 @@ extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double*
     if (delta_final) *delta_final = hst.delta[iters & 1];
     if (delta_zero) *delta_zero = hst.delta0;
     return HS_OK;
+}
+
+extern "C" int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b,
+                                   double eps, int64_t imax, int wg, int wgf, int fma, int64_t* iterations,
+                                   double* delta_final, double* delta_zero) {
+    return cg_ref_run(h, A, rhs, x, N, b, eps, imax, wg, wgf, fma, iterations, delta_final, delta_zero, nullptr);
+}
+
+extern "C" int hs_cg_ref_order_f64_trace(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N,
+                                         int b, double eps, int64_t imax, int wg, int wgf, int fma,
+                                         int64_t* iterations, double* delta_final, double* delta_zero,
+                                         double* trace) {
+    if (h && !trace) return -15;
+    return cg_ref_run(h, A, rhs, x, N, b, eps, imax, wg, wgf, fma, iterations, delta_final, delta_zero, trace);
 }

@@ -238,15 +238,15 @@ static int cholesky_sequential(hs_handle_t h, double* A, i64 N, int b, PhaseTime
     if (rc) return rc;
     for (int k = 0; k < n; ++k) {
         const i64 id = hs_block_id(k, k, n);
-        if ((rc = pt.begin(h->stream, 0))) return rc;
+        if ((rc = pt.begin(h->stream, 0, k))) return rc;
         if ((rc = hs_factor_potrf<double>(h, A, N, b, id, k))) return rc;
         if ((rc = pt.end(h->stream, 0))) return rc;
         const int cnt = n - k - 1;
         if (cnt > 0) {
-            if ((rc = pt.begin(h->stream, 1))) return rc;
+            if ((rc = pt.begin(h->stream, 1, k))) return rc;
             if ((rc = hs_factor_trsm<double>(h, A, b, id, k, k + 1, cnt))) return rc;
             if ((rc = pt.end(h->stream, 1))) return rc;
-            if ((rc = pt.begin(h->stream, 2))) return rc;
+            if ((rc = pt.begin(h->stream, 2, k))) return rc;
             if ((rc = hs_launch_trailing(h, A, b, k, k + 1, cnt, n, 3))) return rc;
             if ((rc = pt.end(h->stream, 2))) return rc;
         }
@@ -290,6 +290,27 @@ extern "C" int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b,
     int rc = cholesky_run(h, A, N, b, pt);
     if (rc < 0) return rc;
     int rc2 = pt.collect(phase_ms, launches);
+    return rc2 ? rc2 : rc;
+}
+
+// Per block column k: the POTRF of diagonal block k, the TRSM of panel k, and
+// the trailing-update launches whose first panel is k (host arrays of n
+// entries, ms); the reference's per-column record (Cholesky.cpp:261-359).
+extern "C" int hs_cholesky_f64_trace(hs_handle_t h, double* A, int64_t N, int b, double* potrf_ms, double* trsm_ms,
+                                     double* update_ms) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (N <= 0) return -3;
+    if (b <= 0) return -4;
+    if (!potrf_ms) return -5;
+    if (!trsm_ms) return -6;
+    if (!update_ms) return -7;
+    PhaseTimer pt;
+    pt.on = true;
+    pt.columns = true;
+    int rc = cholesky_run(h, A, N, b, pt);
+    if (rc < 0) return rc;
+    int rc2 = pt.collect_columns((int)((N + b - 1) / b), potrf_ms, trsm_ms, update_ms);
     return rc2 ? rc2 : rc;
 }
 
@@ -408,6 +429,58 @@ extern "C" int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int
         return HS_OK;
     }
     return tri_solve_blockwise(h, L, x, N, b, trans == 0 ? 1 : 2);
+}
+
+// Host-only executor of the look-ahead schedule: each operation becomes 8
+// int64 values (kind, stream, a0 .. a5) instead of a launch.
+//   0 record(event)        1 wait(event)
+//   2 potrf(k)             3 trsm(k, blockStart, blockCount)
+//   4 trail(k, mode, rowStart, rowCount, np, reads_planes)
+//   5 prep(k0)             (the mixed factor's split of panels k0 .. k0+P-1)
+namespace {
+struct DryRunSchedule {
+    int n, P;
+    bool planes;
+    std::vector<int64_t> ops;
+    void put(int64_t kind, int s, int64_t a0 = 0, int64_t a1 = 0, int64_t a2 = 0, int64_t a3 = 0, int64_t a4 = 0,
+             int64_t a5 = 0) {
+        for (int64_t v : {kind, (int64_t)s, a0, a1, a2, a3, a4, a5}) ops.push_back(v);
+    }
+    int record(int ev, int s) { return put(0, s, ev), HS_OK; }
+    int wait(int s, int ev) { return put(1, s, ev), HS_OK; }
+    int potrf(int k) { return put(2, 2, k), HS_OK; }
+    int trsm(int k, int start, int count) { return put(3, 2, k, start, count), HS_OK; }
+    int trail(int s, int k, int mode, int rowStart, int rowCount, int np) {
+        put(4, s, k, mode, rowStart, rowCount, np, planes && hs_trail_reads_planes(mode, np, P));
+        return HS_OK;
+    }
+    int prep(int k0) {
+        if (planes && hs_plane_prep_needed(k0, P, n)) put(5, 2, k0);
+        return HS_OK;
+    }
+    int begin(int, int, int) { return HS_OK; }
+    int end(int, int) { return HS_OK; }
+};
+}  // namespace
+
+// The look-ahead schedule for n block columns and `panels` panels per step
+// (0: the FP64 factor's default; clamped to 1 .. n/2 as the factors do), with
+// the mixed factor's bf16 plane splits when planes != 0, recorded on the host.
+// *count receives the number of operations; at most `capacity` are written.
+extern "C" int hs_lookahead_dry_run(int n, int panels, int planes, int64_t* ops, int64_t capacity, int64_t* count) {
+    if (n <= 0) return -1;
+    if (!ops && capacity > 0) return -4;
+    if (!count) return -6;
+    DryRunSchedule ex;
+    ex.n = n;
+    ex.P = panels > 0 ? std::max(1, std::min(panels, n / 2)) : hs_schedule_panels(n);
+    ex.planes = planes != 0;
+    int rc = hs_lookahead_schedule(ex, n, ex.P);
+    if (rc) return rc;
+    const int64_t cnt = (int64_t)ex.ops.size() / 8;
+    *count = cnt;
+    for (int64_t i = 0; i < std::min(cnt, capacity) * 8; ++i) ops[i] = ex.ops[(size_t)i];
+    return HS_OK;
 }
 
 // Work of the rest-update launches of the look-ahead schedule the factorisations

@@ -207,18 +207,24 @@ struct OnStream {
 // Per-phase event bracketing (0 = POTRF, 1 = TRSM, 2 = main-stream trailing
 // update), summed after the final synchronisation.
 struct PhaseTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+    // phases 0 POTRF, 1 TRSM, 2 rest updates (S1), 3 the look-ahead column
+    // updates on the panel stream (timed only when `columns` is set); each span
+    // remembers the block column it belongs to (its first panel)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[4];
+    std::vector<int> cols[4];
     bool on = false;
-    int begin(hipStream_t s, int phase) {
-        if (!on) return HS_OK;
+    bool columns = false;
+    int begin(hipStream_t s, int phase, int col = -1) {
+        if (!on || (phase == 3 && !columns)) return HS_OK;
         hipEvent_t e;
         HS_HIP(hipEventCreate(&e));
         HS_HIP(hipEventRecord(e, s));
         spans[phase].push_back({e, nullptr});
+        cols[phase].push_back(col);
         return HS_OK;
     }
     int end(hipStream_t s, int phase) {
-        if (!on) return HS_OK;
+        if (!on || (phase == 3 && !columns)) return HS_OK;
         hipEvent_t e;
         HS_HIP(hipEventCreate(&e));
         HS_HIP(hipEventRecord(e, s));
@@ -236,6 +242,20 @@ struct PhaseTimer {
             ms[p] = t;
             launches[p] = (int64_t)spans[p].size();
         }
+        return HS_OK;
+    }
+    // per block column k < n: POTRF, TRSM, and every trailing-update launch
+    // whose first panel is k (the rest update and the column updates)
+    int collect_columns(int n, double* potrf, double* trsm, double* update) {
+        double* out[4] = {potrf, trsm, update, update};
+        for (int k = 0; k < n; ++k) potrf[k] = trsm[k] = update[k] = 0.0;
+        for (int p = 0; p < 4; ++p)
+            for (size_t i = 0; i < spans[p].size(); ++i) {
+                float v = 0;
+                HS_HIP(hipEventElapsedTime(&v, spans[p][i].first, spans[p][i].second));
+                const int k = cols[p][i];
+                if (k >= 0 && k < n) out[p][k] += v;
+            }
         return HS_OK;
     }
     ~PhaseTimer() {
@@ -289,12 +309,108 @@ inline void hs_rest_update_work(int n, int b, int64_t* launches, double* flops, 
 // soon as they are final, so the next step's panel work runs underneath it.
 // S1's update of step s waits for the panels of step s (event P[s]); S2's
 // column update of step s+1 waits for S1's update of step s-1, which wrote
-// those columns.
-//   potrf(k), trsm(k, blockStart, blockCount): launch on h->stream
-//   trail(stream, k, mode, rowStart, rowCount, np): launch on `stream`
-// No work between a step's panel chain and the event that publishes it.
+// those columns. No work between a step's panel chain and the event that
+// publishes it.
+//
+// The schedule is written once, over an executor that issues its operations
+// on the device (hs_lookahead_factor) or records them on the host
+// (hs_lookahead_dry_run, which the schedule's race analysis replays).
+// Streams are 1 (S1, the handle's ordering stream) and 2 (S2, the panel
+// stream); events are indices, P[s] = s and U[s] = n + 1 + s (2 n + 2 in all):
+//   ex.record(ev, stream), ex.wait(stream, ev)
+//   ex.potrf(k), ex.trsm(k, blockStart, blockCount)      on S2
+//   ex.trail(stream, k, mode, rowStart, rowCount, np)
+//   ex.prep(k0)                                          on S2, after the panel chain of k0
+//   ex.begin(stream, phase, k), ex.end(stream, phase)    timing brackets
+template <class Ex>
+int hs_lookahead_schedule(Ex& ex, int n, int P) {
+    const int U0 = n + 1;
+    int rc;
+    // S2 must see everything already queued on S1 (e.g. the input assembly)
+    if ((rc = ex.record(U0 + n, 1)) || (rc = ex.wait(2, U0 + n))) return rc;
+    // POTRF and TRSM of panels k0 .. k0+P-1 on S2, right-looking inside the
+    // step: panel k updates the step's later columns k+1 .. k0+P-1 at once
+    auto chain = [&](int k0) -> int {
+        int r;
+        const int kend = std::min(n, k0 + P);
+        for (int k = k0; k < kend; ++k) {
+            if ((r = ex.begin(2, 0, k)) || (r = ex.potrf(k)) || (r = ex.end(2, 0))) return r;
+            if (k + 1 < n && ((r = ex.begin(2, 1, k)) || (r = ex.trsm(k, k + 1, n - k - 1)) || (r = ex.end(2, 1))))
+                return r;
+            if (k + 1 < kend && ((r = ex.begin(2, 3, k)) ||
+                                 (r = ex.trail(2, k, k + 2 == kend ? 5 : 6, k + 1, kend - k - 1, 1)) ||
+                                 (r = ex.end(2, 3))))
+                return r;
+        }
+        return HS_OK;
+    };
+    if ((rc = chain(0)) || (rc = ex.prep(0)) || (rc = ex.record(0, 2))) return rc;
+    int step = 0;
+    for (int k = 0; k + P < n; k += P, ++step) {
+        if (step >= 1 && (rc = ex.wait(2, U0 + step - 1))) return rc;
+        // columns k+P .. k+2P-1 by the step's P panels
+        if ((rc = ex.begin(2, 3, k)) || (rc = ex.trail(2, k, P == 1 ? 5 : 6, k + P, P == 1 ? n - k - 1 : P, P)) ||
+            (rc = ex.end(2, 3)))
+            return rc;
+        if ((rc = chain(k + P)) || (rc = ex.prep(k + P)) || (rc = ex.record(step + 1, 2))) return rc;
+        if ((rc = ex.wait(1, step))) return rc;
+        if (k + 2 * P < n) {
+            if ((rc = ex.begin(1, 2, k)) || (rc = ex.trail(1, k, 4, k + 2 * P, n - k - 2 * P, P)) ||
+                (rc = ex.end(1, 2)))
+                return rc;
+        }
+        if ((rc = ex.record(U0 + step, 1))) return rc;
+    }
+    return ex.wait(1, step);
+}
+
+// The mixed factor's bf16 planes: the panels k0 .. k0+P-1 are split after
+// their chain when a later step reads them, and the launches that read the
+// split planes are the step's rest update and its P-panel column update.
+inline bool hs_plane_prep_needed(int k0, int P, int n) { return k0 + P < n; }
+inline bool hs_trail_reads_planes(int mode, int np, int P) {
+    return np == P && (mode == 4 || (mode == 6 && P > 1));
+}
+
 struct NoPanelPrep {
     int operator()(hipStream_t, int) const { return HS_OK; }
+};
+
+// The executor that issues the schedule on the device: potrf(k) and
+// trsm(k, blockStart, blockCount) launch on h->stream, which is S2 while they
+// run; trail(stream, ...) and prep(stream, k0) take their stream.
+template <class Potrf, class Trsm, class Trail, class Prep>
+struct DeviceSchedule {
+    hs_handle_t h;
+    PhaseTimer& pt;
+    Potrf& potrf_;
+    Trsm& trsm_;
+    Trail& trail_;
+    Prep& prep_;
+    hipStream_t s1, s2;
+    hipStream_t str(int s) const { return s == 1 ? s1 : s2; }
+    int record(int ev, int s) {
+        HS_HIP(hipEventRecord(h->events[ev], str(s)));
+        return HS_OK;
+    }
+    int wait(int s, int ev) {
+        HS_HIP(hipStreamWaitEvent(str(s), h->events[ev], 0));
+        return HS_OK;
+    }
+    int potrf(int k) {
+        OnStream os(h, s2);
+        return potrf_(k);
+    }
+    int trsm(int k, int start, int count) {
+        OnStream os(h, s2);
+        return trsm_(k, start, count);
+    }
+    int trail(int s, int k, int mode, int rowStart, int rowCount, int np) {
+        return trail_(str(s), k, mode, rowStart, rowCount, np);
+    }
+    int prep(int k0) { return prep_(s2, k0); }
+    int begin(int s, int phase, int k) { return pt.begin(str(s), phase, k); }
+    int end(int s, int phase) { return pt.end(str(s), phase); }
 };
 
 // panels = 0 takes hs_schedule_panels(n); any other value is clamped to
@@ -307,42 +423,6 @@ int hs_lookahead_factor(hs_handle_t h, int n, PhaseTimer& pt, Potrf potrf, Trsm 
     const int P = panels > 0 ? std::max(1, std::min(panels, n / 2)) : hs_schedule_panels(n);
     int rc = hs_ensure_events(h, 2 * (size_t)n + 2);
     if (rc) return rc;
-    hipEvent_t* evP = h->events.data();          // panels of step s final (S2)
-    hipEvent_t* evU = h->events.data() + n + 1;  // S1 update of step s done
-    hipStream_t s1 = h->stream, s2 = h->stream2;
-    // S2 must see everything already queued on S1 (e.g. the input assembly)
-    HS_HIP(hipEventRecord(evU[n], s1));
-    HS_HIP(hipStreamWaitEvent(s2, evU[n], 0));
-    // POTRF and TRSM of panels k0 .. k0+P-1 on S2, right-looking inside the
-    // step: panel k updates the step's later columns k+1 .. k0+P-1 at once
-    auto chain = [&](int k0) -> int {
-        OnStream os(h, s2);
-        int r;
-        const int kend = std::min(n, k0 + P);
-        for (int k = k0; k < kend; ++k) {
-            if ((r = pt.begin(s2, 0)) || (r = potrf(k)) || (r = pt.end(s2, 0))) return r;
-            if (k + 1 < n && ((r = pt.begin(s2, 1)) || (r = trsm(k, k + 1, n - k - 1)) || (r = pt.end(s2, 1))))
-                return r;
-            if (k + 1 < kend && (r = trail(s2, k, k + 2 == kend ? 5 : 6, k + 1, kend - k - 1, 1))) return r;
-        }
-        return HS_OK;
-    };
-    if ((rc = chain(0)) || (rc = prep(s2, 0))) return rc;
-    HS_HIP(hipEventRecord(evP[0], s2));
-    int step = 0;
-    for (int k = 0; k + P < n; k += P, ++step) {
-        if (step >= 1) HS_HIP(hipStreamWaitEvent(s2, evU[step - 1], 0));
-        // columns k+P .. k+2P-1 by the step's P panels
-        if ((rc = trail(s2, k, P == 1 ? 5 : 6, k + P, P == 1 ? n - k - 1 : P, P))) return rc;
-        if ((rc = chain(k + P)) || (rc = prep(s2, k + P))) return rc;
-        HS_HIP(hipEventRecord(evP[step + 1], s2));
-        HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
-        if (k + 2 * P < n) {
-            if ((rc = pt.begin(s1, 2)) || (rc = trail(s1, k, 4, k + 2 * P, n - k - 2 * P, P)) || (rc = pt.end(s1, 2)))
-                return rc;
-        }
-        HS_HIP(hipEventRecord(evU[step], s1));
-    }
-    HS_HIP(hipStreamWaitEvent(s1, evP[step], 0));
-    return HS_OK;
+    DeviceSchedule<Potrf, Trsm, Trail, Prep> ex{h, pt, potrf, trsm, trail, prep, h->stream, h->stream2};
+    return hs_lookahead_schedule(ex, n, P);
 }

@@ -70,6 +70,53 @@ int hs_ensure_events(hs_handle_t h, size_t count);
 
 #define HS_LAUNCH_CHECK() HS_HIP(hipGetLastError())
 
+// Per-iteration timing of the device-resident CG loops (the traced entries):
+// timing events at five marks of every iteration (before the mat-vec, after
+// it, after alpha and the x update, after the residual, after delta and the d
+// update), held for the two chunks of iterations in flight and read back when
+// the host sees a chunk complete. Iteration i yields out[5 i .. 5 i + 4]:
+// the four phase times and the whole iteration, in ms.
+struct IterTrace {
+    static constexpr int M = 5;
+    double* out = nullptr;
+    i64 cap = 0;
+    int chunk = 1;
+    std::vector<hipEvent_t> ev;
+    int start(double* o, i64 capacity, int chunkIters) {
+        out = o;
+        cap = capacity;
+        chunk = chunkIters;
+        if (!out) return HS_OK;
+        ev.assign((size_t)2 * chunk * M, nullptr);
+        for (auto& e : ev) HS_HIP(hipEventCreate(&e));
+        return HS_OK;
+    }
+    ~IterTrace() {
+        for (auto e : ev)
+            if (e) hipEventDestroy(e);
+    }
+    hipEvent_t at(i64 it, int m) const { return ev[(size_t)((((it / chunk) & 1) * chunk + it % chunk) * M + m)]; }
+    int mark(hipStream_t s, i64 it, int m) {
+        if (!out) return HS_OK;
+        HS_HIP(hipEventRecord(at(it, m), s));
+        return HS_OK;
+    }
+    // iterations [first, last) have completed on the device
+    int read(i64 first, i64 last) {
+        if (!out) return HS_OK;
+        for (i64 it = first; it < last && it < cap; ++it) {
+            float t = 0.f;
+            for (int m = 0; m + 1 < M; ++m) {
+                HS_HIP(hipEventElapsedTime(&t, at(it, m), at(it, m + 1)));
+                out[it * M + m] = t;
+            }
+            HS_HIP(hipEventElapsedTime(&t, at(it, 0), at(it, M - 1)));
+            out[it * M + M - 1] = t;
+        }
+        return HS_OK;
+    }
+};
+
 // packed block ID of (I,J), I >= J
 __host__ __device__ inline i64 hs_block_id(i64 I, i64 J, i64 n) { return J * n - J * (J - 1) / 2 + (I - J); }
 

@@ -610,7 +610,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     }
     auto planesOf = [&](int k) { return planes + ((k / P) & 1) * planeSet; };
     auto prep = [&](hipStream_t s, int k0) -> int {
-        if (!planes || k0 + P >= n) return HS_OK;
+        if (!planes || !hs_plane_prep_needed(k0, P, n)) return HS_OK;
         return split_panels(s, L32, b, n, k0, P, k0 + P, n - (k0 + P), planesOf(k0));
     };
     auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
@@ -619,7 +619,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
         // the step's rest update and its P-panel column update read the split planes
-        const bool shared = planes && np == P && (mode == 4 || (mode == 6 && P > 1));
+        const bool shared = planes && hs_trail_reads_planes(mode, np, P);
         return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np,
                                         shared ? planesOf(k) : nullptr, k + P);
     };

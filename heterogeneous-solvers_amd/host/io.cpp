@@ -70,21 +70,52 @@ RightHandSide MatrixParser::parseRightHandSide(std::string& path, hs::Queue& que
     return b;
 }
 
+// One value as the reference writes it: fixed notation, a leading space for
+// values >= 0 (so columns line up with negative ones), ';' after each.
+static void putValue(std::ostream& out, double v) {
+    if (v >= 0) out << ' ';
+    out << v << ';';
+}
+
+// All n b rows and columns (padding included), fixed with 20 digits: row i
+// takes blocks J <= I row-wise, then blocks J > I from the stored (J, I)
+// transposed (MatrixParser.cpp:214-274). The file is `path` itself.
 void MatrixParser::writeFullMatrix(const std::string& path, const SymmetricMatrix& A) {
-    std::ofstream out(path + ".txt");
-    out << std::setprecision(20) << std::scientific;
-    out << "# " << A.N << "\n";
-    const int b = A.blockSize;
-    for (std::size_t i = 0; i < A.N; ++i) {
-        const int I = static_cast<int>(i / b), il = static_cast<int>(i % b);
-        for (std::size_t j = 0; j < A.N; ++j) {
-            const int J = static_cast<int>(j / b), jl = static_cast<int>(j % b);
-            double v;
-            if (J <= I) v = A.matrixData[A.blockID(I, J) * b * b + static_cast<std::size_t>(il) * b + jl];
-            else v = A.matrixData[A.blockID(J, I) * b * b + static_cast<std::size_t>(jl) * b + il];
-            out << (j ? ";" : "") << v;
+    std::ofstream out(path);
+    out << std::setprecision(20) << std::fixed;
+    const int b = A.blockSize, n = A.blockCountXY;
+    const std::size_t bb = static_cast<std::size_t>(b) * b;
+    for (int row = 0; row < n * b; ++row) {
+        const int I = row / b, il = row % b;
+        for (int J = 0; J <= I; ++J) {
+            const double* blk = A.matrixData.data() + A.blockID(I, J) * bb + static_cast<std::size_t>(il) * b;
+            for (int j = 0; j < b; ++j) putValue(out, blk[j]);
         }
-        out << "\n";
+        for (int J = I + 1; J < n; ++J) {
+            const double* blk = A.matrixData.data() + A.blockID(J, I) * bb;
+            for (int j = 0; j < b; ++j) putValue(out, blk[static_cast<std::size_t>(j) * b + il]);
+        }
+        out << std::endl;
+    }
+}
+
+// The stored blocks of each row's block row, fixed with 10 digits, a tab
+// after each block and an empty line before each block row
+// (MatrixParser.cpp:170-211).
+void MatrixParser::writeBlockedMatrix(const std::string& path, const SymmetricMatrix& A) {
+    std::ofstream out(path);
+    out << std::setprecision(10) << std::fixed;
+    const int b = A.blockSize, n = A.blockCountXY;
+    const std::size_t bb = static_cast<std::size_t>(b) * b;
+    for (int row = 0; row < n * b; ++row) {
+        if (row % b == 0) out << std::endl;
+        const int I = row / b, il = row % b;
+        for (int J = 0; J <= I; ++J) {
+            const double* blk = A.matrixData.data() + A.blockID(I, J) * bb + static_cast<std::size_t>(il) * b;
+            for (int j = 0; j < b; ++j) putValue(out, blk[j]);
+            out << '\t';
+        }
+        out << std::endl;
     }
 }
 

@@ -93,71 +93,82 @@ void MetricsTracker::updateMetrics(std::size_t, std::size_t blockCount_GPU, std:
     iterationTimes.push_back(iterationTime);
 }
 
+// vectorToJSONString (MetricsTracker.cpp:309-325): "[a, b, c]" with each
+// value through std::to_string (fixed, six decimals)
 template <typename T>
 static std::string toJSON(const std::vector<T>& v) {
-    std::ostringstream s;
-    s << std::setprecision(17) << "[";
-    for (std::size_t i = 0; i < v.size(); ++i) s << (i ? "," : "") << v[i];
-    s << "]";
-    return s.str();
+    std::string s = "[";
+    for (std::size_t i = 0; i < v.size(); ++i) s += (i ? ", " : "") + std::to_string(v[i]);
+    return s + "]";
 }
 
+// MetricsTracker::writeJSON (MetricsTracker.cpp:152-307) as built without the
+// hws sampler: the configuration block, the common runtime metrics, then the
+// CG or the Cholesky vectors; scalars through std::to_string.
 void MetricsTracker::writeJSON(std::string& path) {
     std::ofstream f(path + "/metrics.json");
+    auto kv = [&](const char* key, const std::string& value, bool last = false) {
+        f << "\t \"" << key << "\": " << value << (last ? "\n" : ",\n");
+    };
+    auto str = [](const std::string& v) { return "\"" + v + "\""; };
     f << "{\n\"configuration\": {\n";
-    f << "\t \"algorithm\": \"" << conf::algorithm << "\",\n";
-    f << "\t \"N\": " << conf::N << ",\n";
-    f << "\t \"FP_type\": \"FP_64\",\n";
-    f << "\t \"matrixBlockSize\": " << conf::matrixBlockSize << ",\n";
-    f << "\t \"workGroupSize\": " << conf::workGroupSize << ",\n";
-    f << "\t \"workGroupSizeVector\": " << conf::workGroupSizeVector << ",\n";
-    f << "\t \"workGroupSizeFinalScalarProduct\": " << conf::workGroupSizeFinalScalarProduct << ",\n";
-    f << "\t \"iMax\": " << conf::iMax << ",\n";
-    f << "\t \"epsilon\": " << conf::epsilon << ",\n";
-    f << "\t \"updateInterval\": " << conf::updateInterval << ",\n";
-    f << "\t \"initialProportionGPU\": " << conf::initialProportionGPU << ",\n";
-    f << "\t \"runtimeLBFactorCPU\": " << conf::runtimeLBFactorCPU << ",\n";
-    f << "\t \"blockUpdateThreshold\": " << conf::blockUpdateThreshold << ",\n";
-    f << "\t \"gpuOptimizationLevel\": " << conf::gpuOptimizationLevel << ",\n";
-    f << "\t \"cpuOptimizationLevel\": " << conf::cpuOptimizationLevel << ",\n";
-    f << "\t \"enableHWS\": " << conf::enableHWS << ",\n";
+    kv("algorithm", str(conf::algorithm));
+    kv("N", std::to_string(conf::N));
+    kv("FP_type", str("FP_64"));
+    kv("matrixBlockSize", std::to_string(conf::matrixBlockSize));
+    kv("workGroupSize", std::to_string(conf::workGroupSize));
+    kv("workGroupSizeVector", std::to_string(conf::workGroupSizeVector));
+    kv("workGroupSizeFinalScalarProduct", std::to_string(conf::workGroupSizeFinalScalarProduct));
+    kv("iMax", std::to_string(conf::iMax));
+    kv("epsilon", std::to_string(conf::epsilon));
+    kv("updateInterval", std::to_string(conf::updateInterval));
+    kv("initialProportionGPU", std::to_string(conf::initialProportionGPU));
+    kv("runtimeLBFactorCPU", std::to_string(conf::runtimeLBFactorCPU));
+    kv("blockUpdateThreshold", std::to_string(conf::blockUpdateThreshold));
+    kv("gpuOptimizationLevel", std::to_string(conf::gpuOptimizationLevel));
+    kv("cpuOptimizationLevel", std::to_string(conf::cpuOptimizationLevel));
+    kv("enableHWS", std::to_string(conf::enableHWS));
     const char* bind = std::getenv("OMP_PROC_BIND");
     const char* cores = std::getenv("OMP_NUM_THREADS");
-    f << "\t \"cpuBind\": \"" << (bind ? bind : "") << "\",\n";
-    f << "\t \"ompCores\": " << (cores ? cores : "\"\"") << ",\n";
-    f << "\t \"mode\": \"" << conf::mode << "\"\n";
+    kv("cpuBind", str(bind ? bind : ""));
+    kv("ompCores", cores ? std::string(cores) : std::string("\"\""));
+    kv("mode", str(conf::mode), true);
     f << "},\n\"runtime-metrics\": {\n";
-    f << "\t \"iterationTimes\": " << toJSON(iterationTimes) << ",\n";
-    f << "\t \"memoryInitTime\": " << memoryInitTime << ",\n";
-    f << "\t \"resultCopyTime\": " << resultCopyTime << ",\n";
-    f << "\t \"totalTime\": " << totalTime << ",\n";
-    f << "\t \"choleskySolveStepTime\": " << solveTime << ",\n";
-    f << "\t \"averageUtilization_GPU\": " << toJSON(averageUtilization_GPU) << ",\n";
-    f << "\t \"averageUtilization_CPU\": " << toJSON(averageUtilization_CPU) << ",\n";
-    f << "\t \"powerDraw_GPU\": " << toJSON(powerDraw_GPU) << ",\n";
-    f << "\t \"powerDraw_CPU\": " << toJSON(powerDraw_CPU) << ",\n";
-    f << "\t \"blockCounts_GPU\": " << toJSON(blockCounts_GPU) << ",\n";
-    f << "\t \"blockCounts_CPU\": " << toJSON(blockCounts_CPU) << ",\n";
-    f << "\t \"matrixVectorTimes_GPU\": " << toJSON(matrixVectorTimes_GPU) << ",\n";
-    f << "\t \"matrixVectorTimes_CPU\": " << toJSON(matrixVectorTimes_CPU) << ",\n";
-    f << "\t \"times_q\": " << toJSON(times_q) << ",\n";
-    f << "\t \"times_alpha\": " << toJSON(times_alpha) << ",\n";
-    f << "\t \"times_x\": " << toJSON(times_x) << ",\n";
-    f << "\t \"times_r\": " << toJSON(times_r) << ",\n";
-    f << "\t \"times_delta\": " << toJSON(times_delta) << ",\n";
-    f << "\t \"times_d\": " << toJSON(times_d) << ",\n";
-    f << "\t \"memcopy_d\": " << toJSON(memcopy_d) << ",\n";
-    f << "\t \"shiftTimes\": " << toJSON(shiftTimes) << ",\n";
-    f << "\t \"choleskyDiagonalBlockTimes\": " << toJSON(choleskyDiagonalBlockTimes) << ",\n";
-    f << "\t \"copyTimes\": " << toJSON(copyTimes) << ",\n";
-    f << "\t \"triangularSolveTimes_GPU\": " << toJSON(triangularSolveTimes_GPU) << ",\n";
-    f << "\t \"triangularSolveTimes_CPU\": " << toJSON(triangularSolveTimes_CPU) << ",\n";
-    f << "\t \"triangularSolveTimes_total\": " << toJSON(triangularSolveTimes_total) << ",\n";
-    f << "\t \"matrixMatrixDiagonalTimes_GPU\": " << toJSON(matrixMatrixDiagonalTimes_GPU) << ",\n";
-    f << "\t \"matrixMatrixDiagonalTimes_CPU\": " << toJSON(matrixMatrixDiagonalTimes_CPU) << ",\n";
-    f << "\t \"matrixMatrixDiagonalTimes_total\": " << toJSON(matrixMatrixDiagonalTimes_total) << ",\n";
-    f << "\t \"matrixMatrixTimes_GPU\": " << toJSON(matrixMatrixTimes_GPU) << ",\n";
-    f << "\t \"matrixMatrixTimes_CPU\": " << toJSON(matrixMatrixTimes_CPU) << ",\n";
-    f << "\t \"matrixMatrixTimes_total\": " << toJSON(matrixMatrixTimes_total) << "\n";
+    kv("iterationTimes", toJSON(iterationTimes));
+    kv("memoryInitTime", std::to_string(memoryInitTime));
+    kv("resultCopyTime", std::to_string(resultCopyTime));
+    kv("totalTime", std::to_string(totalTime));
+    kv("choleskySolveStepTime", std::to_string(solveTime));
+    kv("averageUtilization_GPU", toJSON(averageUtilization_GPU));
+    kv("averageUtilization_CPU", toJSON(averageUtilization_CPU));
+    kv("powerDraw_GPU", toJSON(powerDraw_GPU));
+    kv("powerDraw_CPU", toJSON(powerDraw_CPU));
+    kv("blockCounts_GPU", toJSON(blockCounts_GPU));
+    const bool cg = conf::algorithm == "cg", chol = conf::algorithm == "cholesky";
+    kv("blockCounts_CPU", toJSON(blockCounts_CPU), !cg && !chol);
+    if (cg) {
+        kv("matrixVectorTimes_GPU", toJSON(matrixVectorTimes_GPU));
+        kv("matrixVectorTimes_CPU", toJSON(matrixVectorTimes_CPU));
+        kv("times_q", toJSON(times_q));
+        kv("times_alpha", toJSON(times_alpha));
+        kv("times_x", toJSON(times_x));
+        kv("times_r", toJSON(times_r));
+        kv("times_delta", toJSON(times_delta));
+        kv("times_d", toJSON(times_d));
+        kv("memcopy_d", toJSON(memcopy_d), true);
+    } else if (chol) {
+        kv("shiftTimes", toJSON(shiftTimes));
+        kv("choleskyDiagonalBlockTimes", toJSON(choleskyDiagonalBlockTimes));
+        kv("copyTimes", toJSON(copyTimes));
+        kv("triangularSolveTimes_GPU", toJSON(triangularSolveTimes_GPU));
+        kv("triangularSolveTimes_CPU", toJSON(triangularSolveTimes_CPU));
+        kv("triangularSolveTimes_total", toJSON(triangularSolveTimes_total));
+        kv("matrixMatrixDiagonalTimes_GPU", toJSON(matrixMatrixDiagonalTimes_GPU));
+        kv("matrixMatrixDiagonalTimes_CPU", toJSON(matrixMatrixDiagonalTimes_CPU));
+        kv("matrixMatrixDiagonalTimes_total", toJSON(matrixMatrixDiagonalTimes_total));
+        kv("matrixMatrixTimes_GPU", toJSON(matrixMatrixTimes_GPU));
+        kv("matrixMatrixTimes_CPU", toJSON(matrixMatrixTimes_CPU));
+        kv("matrixMatrixTimes_total", toJSON(matrixMatrixTimes_total), true);
+    }
     f << "}\n}\n";
 }

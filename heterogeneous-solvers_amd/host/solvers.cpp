@@ -81,25 +81,40 @@ void CG::solveHeterogeneous() {
     t0 = clk::now();
     int64_t it = 0;
     double df = 0.0, d0 = 0.0;
+    // per-iteration phase times from device events (5 per iteration)
+    std::vector<double> trace(5 * static_cast<std::size_t>(conf::iMax), 0.0);
     if (conf::referenceOrder) {
-        hs::check(hs_cg_ref_order_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize,
-                                      conf::epsilon, static_cast<int64_t>(conf::iMax), conf::workGroupSizeVector,
-                                      conf::workGroupSizeFinalScalarProduct, conf::referenceOrderFMA ? 1 : 0, &it,
-                                      &df, &d0),
-                  "hs_cg_ref_order_f64");
+        hs::check(hs_cg_ref_order_f64_trace(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N),
+                                            A.blockSize, conf::epsilon, static_cast<int64_t>(conf::iMax),
+                                            conf::workGroupSizeVector, conf::workGroupSizeFinalScalarProduct,
+                                            conf::referenceOrderFMA ? 1 : 0, &it, &df, &d0, trace.data()),
+                  "hs_cg_ref_order_f64_trace");
     } else {
-        hs::check(hs_cg_f64(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize,
-                            conf::epsilon, static_cast<int64_t>(conf::iMax), &it, &df, &d0),
-                  "hs_cg_f64");
+        hs::check(hs_cg_f64_trace(h, dA.data(), db.data(), dx.data(), static_cast<int64_t>(A.N), A.blockSize,
+                                  conf::epsilon, static_cast<int64_t>(conf::iMax), &it, &df, &d0, trace.data()),
+                  "hs_cg_f64_trace");
     }
-    const double solveMs = ms_since(t0);
     iterations = static_cast<std::size_t>(it);
     deltaZero = d0;
     deltaFinal = df;
-    // the device loop has no per-iteration host timing: record the mean
-    for (std::size_t i = 0; i < iterations; ++i)
-        metricsTracker.updateMetrics(i, A.blockCountXY, 0, solveMs / static_cast<double>(iterations),
-                                     loadBalancer->updateInterval);
+    // The reference records per iteration (CG.cpp:109-129): the iteration
+    // time with its block counts, the six step times and the mat-vec time.
+    // Here alpha and delta are fused into the x and d kernels: their times
+    // are inside times_x / times_d, and times_alpha / times_delta hold 0. At
+    // share 1.0 no d exchange happens: memcopy_d holds 0.
+    for (std::size_t i = 0; i < iterations; ++i) {
+        const double* t = &trace[5 * i];
+        metricsTracker.updateMetrics(i, A.blockCountXY, 0, t[4], loadBalancer->updateInterval);
+        metricsTracker.times_q.push_back(t[0]);
+        metricsTracker.times_alpha.push_back(0.0);
+        metricsTracker.times_x.push_back(t[1]);
+        metricsTracker.times_r.push_back(t[2]);
+        metricsTracker.times_delta.push_back(0.0);
+        metricsTracker.times_d.push_back(t[3]);
+        metricsTracker.memcopy_d.push_back(0.0);
+        metricsTracker.matrixVectorTimes_GPU.push_back(t[0]);
+        metricsTracker.matrixVectorTimes_CPU.push_back(0.0);
+    }
 
     t0 = clk::now();
     hs::check(hs_memcpy_dtoh(h, x.data(), dx.data(), bytesV), "hs_memcpy_dtoh");
@@ -137,15 +152,35 @@ void Cholesky::solve_heterogeneous() {
     hs::check(hs_memcpy_htod(h, A_gpu, A.matrixData.data(), bytes), "hs_memcpy_htod");
     metricsTracker.memoryInitTime = ms_since(t0);
 
-    double phase[3] = {0, 0, 0};
-    int64_t launches[3] = {0, 0, 0};
-    const int rc = hs_cholesky_f64_timed(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize, phase, launches);
+    const std::size_t n = static_cast<std::size_t>(A.blockCountXY);
+    std::vector<double> potrf(n), trsm(n), update(n);
+    const int rc = hs_cholesky_f64_trace(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize, potrf.data(), trsm.data(),
+                                         update.data());
     hs::check(rc, "hs_cholesky_f64");
     info = rc;
     if (info > 0) std::cerr << "warning: non-positive pivot at row " << info << " (matrix not SPD)" << std::endl;
-    metricsTracker.choleskyDiagonalBlockTimes.push_back(phase[0]);
-    metricsTracker.triangularSolveTimes_GPU.push_back(phase[1]);
-    metricsTracker.matrixMatrixTimes_GPU.push_back(phase[2]);
+    // The reference's per-column record (Cholesky.cpp:261-359). SYRK and GEMM
+    // are one fused launch here, recorded under matrixMatrixTimes (the
+    // diagonal-update vectors hold 0); a launch that applies several panels is
+    // recorded at its first column. No rows are shifted or copied between
+    // devices at share 1.0: shiftTimes and copyTimes hold 0. The column time
+    // is the sum of the column's kernel times (the panel and update streams
+    // overlap, so these add up to more than the factorisation's wall time).
+    for (std::size_t k = 0; k < n; ++k) {
+        metricsTracker.shiftTimes.push_back(0.0);
+        metricsTracker.choleskyDiagonalBlockTimes.push_back(potrf[k]);
+        metricsTracker.copyTimes.push_back(0.0);
+        metricsTracker.triangularSolveTimes_GPU.push_back(trsm[k]);
+        metricsTracker.triangularSolveTimes_CPU.push_back(0.0);
+        metricsTracker.triangularSolveTimes_total.push_back(trsm[k]);
+        metricsTracker.matrixMatrixDiagonalTimes_GPU.push_back(0.0);
+        metricsTracker.matrixMatrixDiagonalTimes_CPU.push_back(0.0);
+        metricsTracker.matrixMatrixDiagonalTimes_total.push_back(0.0);
+        metricsTracker.matrixMatrixTimes_GPU.push_back(update[k]);
+        metricsTracker.matrixMatrixTimes_CPU.push_back(0.0);
+        metricsTracker.matrixMatrixTimes_total.push_back(update[k]);
+        metricsTracker.updateMetrics(k, n, 0, potrf[k] + trsm[k] + update[k], loadBalancer->updateInterval);
+    }
 
     t0 = clk::now();
     hs::check(hs_memcpy_dtoh(h, A.matrixData.data(), A_gpu, bytes), "hs_memcpy_dtoh");

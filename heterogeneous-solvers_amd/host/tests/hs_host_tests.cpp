@@ -8,6 +8,9 @@
 //   lb <share>                           StaticLoadBalancer validation
 //   cgdims                               CG dimension checks
 //   gpcpu                                matrixVectorGP on a CPU queue (rejected)
+//   writefull <matrix.txt> <bsz> <out>   MatrixParser::writeFullMatrix of the parsed matrix
+//   writeblocked <matrix.txt> <bsz> <out> MatrixParser::writeBlockedMatrix of it
+//   metrics <algorithm> <dir> <count>    MetricsTracker::writeJSON with <count> entries per vector
 // and, on a GPU:
 //   gemvgp <m> <n>                       matrixVectorGP on exact binary data
 //   kstar <train> <test> <N> <M>         generateTestKernelMatrix, all entries
@@ -84,6 +87,37 @@ int main(int argc, char** argv) {
             } catch (const std::invalid_argument& e) {
                 std::cout << "invalid_argument " << e.what() << std::endl;
             }
+            return 0;
+        }
+        if (cmd == "writefull" || cmd == "writeblocked") {
+            std::string path = argv[2];
+            conf::matrixBlockSize = std::stoul(argv[3]);
+            SymmetricMatrix A = MatrixParser::parseSymmetricMatrix(path, host);
+            if (cmd == "writefull") MatrixParser::writeFullMatrix(argv[4], A);
+            else MatrixParser::writeBlockedMatrix(argv[4], A);
+            std::cout << "ok" << std::endl;
+            return 0;
+        }
+        if (cmd == "metrics") {
+            conf::algorithm = argv[2];
+            std::string dir = argv[3];
+            const int count = std::stoi(argv[4]);
+            MetricsTracker m;
+            for (int i = 0; i < count; ++i) {
+                const double t = 0.5 + i;
+                m.updateMetrics(i, 4, 0, t, conf::updateInterval);
+                for (auto* v : {&m.times_q, &m.times_alpha, &m.times_x, &m.times_r, &m.times_delta, &m.times_d,
+                                &m.memcopy_d, &m.matrixVectorTimes_GPU, &m.matrixVectorTimes_CPU, &m.shiftTimes,
+                                &m.choleskyDiagonalBlockTimes, &m.copyTimes, &m.triangularSolveTimes_GPU,
+                                &m.triangularSolveTimes_CPU, &m.triangularSolveTimes_total,
+                                &m.matrixMatrixDiagonalTimes_GPU, &m.matrixMatrixDiagonalTimes_CPU,
+                                &m.matrixMatrixDiagonalTimes_total, &m.matrixMatrixTimes_GPU,
+                                &m.matrixMatrixTimes_CPU, &m.matrixMatrixTimes_total})
+                    v->push_back(t / 4);
+            }
+            m.totalTime = 12.25;
+            m.writeJSON(dir);
+            std::cout << "ok" << std::endl;
             return 0;
         }
         if (cmd == "gemvgp") {

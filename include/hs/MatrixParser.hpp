@@ -15,5 +15,9 @@ public:
     static SymmetricMatrix parseSymmetricMatrix(std::string& path, hs::Queue& queue);
     static RightHandSide parseRightHandSide(std::string& path, hs::Queue& queue);
     static std::vector<conf::fp_type> getRowValuesFromString(const std::string& rowString);
+    // the full symmetric matrix, padding included, one row per line
+    // (MatrixParser.cpp:214-274)
     static void writeFullMatrix(const std::string& path, const SymmetricMatrix& matrix);
+    // the stored lower block triangle by block rows (MatrixParser.cpp:170-211)
+    static void writeBlockedMatrix(const std::string& path, const SymmetricMatrix& matrix);
 };

@@ -184,6 +184,25 @@ int hs_cholesky_f64_timed(hs_handle_t h, double* A, int64_t N, int b, double* ph
  * for N and b (b a multiple of 64): their number and
  * their algorithmic flops (b^3 per diagonal block and 2 b^3 per off-diagonal
  * block and panel). Measurement helper; no reference counterpart. */
+/* hs_cholesky_f64 with a per-column timing record (host arrays of n =
+ * ceil(N/b) doubles, ms): potrf_ms[k] the diagonal block k, trsm_ms[k] panel
+ * k, update_ms[k] the fused SYRK+GEMM trailing-update launches whose first
+ * panel is k (the look-ahead schedule updates several panels per launch, so
+ * the other columns of a step hold 0). The per-column vectors of
+ * Cholesky::solve_heterogeneous's metrics (Cholesky.cpp:261-359). */
+int hs_cholesky_f64_trace(hs_handle_t h, double* A, int64_t N, int b, double* potrf_ms, double* trsm_ms,
+                          double* update_ms);
+/* The two-stream look-ahead schedule of the factorisations, recorded on the
+ * host instead of issued (no device needed): n block columns, `panels` panels
+ * per step (0 = the FP64 factor's default, clamped to 1 .. n/2), the mixed
+ * factor's bf16 plane splits when planes != 0. Each operation is 8 int64
+ * (kind, stream, a0 .. a5): 0 record(event), 1 wait(event), 2 potrf(k),
+ * 3 trsm(k, start, count), 4 trail(k, mode, rowStart, rowCount, np,
+ * reads_planes), 5 prep(k0); streams 1 (ordering) and 2 (panels); events
+ * P[s] = s, U[s] = n + 1 + s. No reference counterpart: the reference's
+ * schedule is the sequence with a host wait after each kernel
+ * (Cholesky.cpp:163-259). */
+int hs_lookahead_dry_run(int n, int panels, int planes, int64_t* ops, int64_t capacity, int64_t* count);
 int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops);
 
 /* The same for the look-ahead schedule of hs_cholesky_mixed with this variant
@@ -219,6 +238,15 @@ int hs_trsv_packed_f64(hs_handle_t h, const double* L, double* x, int64_t N, int
 int hs_cg_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
               int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero);
 
+/* hs_cg_f64 with a per-iteration timing record: trace (host, 5 imax doubles)
+ * receives for iteration i, in ms, [5i] the mat-vec q = A d, [5i+1] alpha, the
+ * x update and the recursive residual update (one fused kernel), [5i+2] the
+ * true residual (iterations % 50 == 0, else ~0), [5i+3] delta_new, beta and
+ * the d update (fused), [5i+4] the whole iteration. The per-iteration vectors
+ * of CG::solveHeterogeneous's metrics (CG.cpp:62-129). */
+int hs_cg_f64_trace(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
+                    int64_t imax, int64_t* iterations, double* delta_final, double* delta_zero, double* trace);
+
 /* hs_cg_f64 without any allocation and without the handle workspace: no
  * counterpart in the reference; for callers that cannot allocate device
  * memory (graph capture, serving loops). Its scratch (SYMV partials, CG
@@ -246,6 +274,12 @@ int hs_cg_noalloc_limit(size_t* scratch_bytes);
 int hs_cg_ref_order_f64(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b, double eps,
                         int64_t imax, int wg, int wgf, int fma, int64_t* iterations, double* delta_final,
                         double* delta_zero);
+/* hs_cg_ref_order_f64 with the per-iteration record of hs_cg_f64_trace;
+ * [5i+1] is the d.q product, alpha and the x update, [5i+2] the residual
+ * (true or recursive), [5i+3] the r.r product, beta and the d update. */
+int hs_cg_ref_order_f64_trace(hs_handle_t h, const double* A, const double* rhs, double* x, int64_t N, int b,
+                              double eps, int64_t imax, int wg, int wgf, int fma, int64_t* iterations,
+                              double* delta_final, double* delta_zero, double* trace);
 /* The parity mode's mat-vec alone, y = A x over all n block rows
  * (matrixVectorBlock_GPU over the whole matrix, reset = true). */
 int hs_symv_ref_order_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b, int fma);

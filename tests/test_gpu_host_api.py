@@ -30,6 +30,26 @@ def result(r, key):
     return float(m.group(1))
 
 
+def metrics(tmp_path):
+    """The run's metrics.json (written under --output/<time stamp>/)."""
+    import glob
+    import json
+
+    files = glob.glob(str(tmp_path / "out" / "*" / "metrics.json"))
+    assert len(files) == 1, files
+    return json.load(open(files[0]))
+
+
+def check_x_result(path, n_values):
+    """UtilityFunctions::writeResult (UtilityFunctions.cpp:19-33): one line,
+    every value in fixed notation with 20 decimals followed by a space."""
+    text = open(path).read()
+    assert text.endswith(" \n") and text.count("\n") == 1
+    toks = text[:-2].split(" ")
+    assert len(toks) == n_values
+    assert all(re.fullmatch(r"-?\d+\.\d{20}", t) for t in toks)
+
+
 @pytest.mark.parametrize("bsz", [4, 6, 20])
 def test_driver_cholesky_on_reference_fixture(tmp_path, bsz):
     r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", bsz, "--init_gpu_perc", 1,
@@ -39,7 +59,15 @@ def test_driver_cholesky_on_reference_fixture(tmp_path, bsz):
     ref = np.linalg.solve(fixture_matrix(), fixture_vector())
     assert np.allclose(x, ref, rtol=1e-10, atol=1e-12)
     assert result(r, "relative_residual") < 1e-12
-    assert os.path.exists(tmp_path / "out")
+    n = -(-20 // bsz)
+    check_x_result(tmp_path / "x_result.txt", n * bsz)
+    # per-column records of the factorisation (Cholesky.cpp:261-359)
+    rt = metrics(tmp_path)["runtime-metrics"]
+    for k in ("iterationTimes", "choleskyDiagonalBlockTimes", "triangularSolveTimes_GPU", "matrixMatrixTimes_GPU",
+              "blockCounts_GPU"):
+        assert len(rt[k]) == n, k
+    assert all(v > 0 for v in rt["choleskyDiagonalBlockTimes"])
+    assert rt["choleskySolveStepTime"] > 0
 
 
 def _gp_files(tmp_path, N, seed=7):
@@ -113,6 +141,15 @@ def test_driver_cg_reference_order_iterations(tmp_path, oracle, fma):
     it = int(result(r, "iterations"))
     print("reference order fma=%d: iterations driver %d oracle %d" % (fma, it, itref), flush=True)
     assert abs(it - itref) <= 1
+    # one record per iteration (CG.cpp:109-129), measured on the device
+    rt = metrics(tmp_path)["runtime-metrics"]
+    for k in ("iterationTimes", "times_q", "times_x", "times_r", "times_d", "matrixVectorTimes_GPU",
+              "blockCounts_GPU"):
+        assert len(rt[k]) == it, k
+    assert all(v > 0 for v in rt["times_q"]) and all(v > 0 for v in rt["iterationTimes"])
+    assert all(t >= q for t, q in zip(rt["iterationTimes"], rt["times_q"]))
+    # the true residual of iterations % 50 == 0 is the only mat-vec in times_r
+    assert rt["times_r"][0] > 0.5 * rt["times_q"][0]
     x = np.loadtxt(tmp_path / "x_result.txt")
     if it == itref:
         assert np.linalg.norm(x - xref) <= 1e-12 * np.linalg.norm(xref)

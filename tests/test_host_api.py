@@ -2,8 +2,10 @@
 the text parser and packed layout against the reference's
 testSymmetricMatrix.cpp goldens, load-balancer and CG argument checks."""
 import os
+import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from golden_cases import GOLDENS
@@ -70,3 +72,98 @@ def test_gp_matvec_rejects_cpu_queue():
     not a silent host fallback."""
     out = run("gpcpu")
     assert out[0].startswith("invalid_argument matrixVectorGP")
+
+
+def _dense_padded(b):
+    from hs_testlib import fixture_matrix, nblocks
+    M = fixture_matrix()
+    n = nblocks(20, b)
+    P = np.zeros((n * b, n * b))
+    P[:20, :20] = M
+    return P
+
+
+@pytest.mark.parametrize("b", [4, 6, 20])
+def test_write_full_matrix_round_trip(tmp_path, b):
+    """MatrixParser::writeFullMatrix (MatrixParser.cpp:214-274): every padded
+    row, values in fixed notation with 20 decimals, ' ' before a value >= 0,
+    ';' after each. Read back, the values are the parsed fixture's, bit for
+    bit (20 decimals keep every double of this fixture)."""
+    out = tmp_path / "A_full"
+    assert run("writefull", MATRIX, b, out)[0] == "ok"
+    lines = out.read_text().split("\n")
+    assert lines[-1] == ""
+    P = _dense_padded(b)
+    rows = lines[:-1]
+    assert len(rows) == P.shape[0]
+    for i, row in enumerate(rows):
+        toks = row.split(";")
+        assert toks[-1] == ""
+        toks = toks[:-1]
+        assert len(toks) == P.shape[1]
+        for j, t in enumerate(toks):
+            assert re.fullmatch(r"( \d+|-\d+)\.\d{20}", t), t
+            assert float(t) == P[i, j], (i, j, t, P[i, j])
+
+
+def test_write_blocked_matrix_layout(tmp_path):
+    """MatrixParser::writeBlockedMatrix (MatrixParser.cpp:170-211): an empty
+    line before each block row, the stored blocks J <= I of each row with a
+    tab after each block, 10 decimals."""
+    b = 6
+    out = tmp_path / "A_blocked"
+    assert run("writeblocked", MATRIX, b, out)[0] == "ok"
+    lines = out.read_text().split("\n")
+    P = _dense_padded(b)
+    n = P.shape[0] // b
+    i = 0
+    for I in range(n):
+        assert lines[i] == ""
+        i += 1
+        for il in range(b):
+            blocks = lines[i].split("\t")
+            assert blocks[-1] == "" and len(blocks) == I + 2
+            for J, blk in enumerate(blocks[:-1]):
+                vals = [float(t) for t in blk.split(";")[:-1]]
+                assert len(vals) == b
+                assert np.allclose(vals, P[I * b + il, J * b:(J + 1) * b], atol=5e-11, rtol=0)
+            i += 1
+
+
+# MetricsTracker::writeJSON without the hws sampler (MetricsTracker.cpp:179-275)
+CONFIG_KEYS = ["algorithm", "N", "FP_type", "matrixBlockSize", "workGroupSize", "workGroupSizeVector",
+               "workGroupSizeFinalScalarProduct", "iMax", "epsilon", "updateInterval", "initialProportionGPU",
+               "runtimeLBFactorCPU", "blockUpdateThreshold", "gpuOptimizationLevel", "cpuOptimizationLevel",
+               "enableHWS", "cpuBind", "ompCores", "mode"]
+COMMON_KEYS = ["iterationTimes", "memoryInitTime", "resultCopyTime", "totalTime", "choleskySolveStepTime",
+               "averageUtilization_GPU", "averageUtilization_CPU", "powerDraw_GPU", "powerDraw_CPU",
+               "blockCounts_GPU", "blockCounts_CPU"]
+CG_KEYS = ["matrixVectorTimes_GPU", "matrixVectorTimes_CPU", "times_q", "times_alpha", "times_x", "times_r",
+           "times_delta", "times_d", "memcopy_d"]
+CHOL_KEYS = ["shiftTimes", "choleskyDiagonalBlockTimes", "copyTimes", "triangularSolveTimes_GPU",
+             "triangularSolveTimes_CPU", "triangularSolveTimes_total", "matrixMatrixDiagonalTimes_GPU",
+             "matrixMatrixDiagonalTimes_CPU", "matrixMatrixDiagonalTimes_total", "matrixMatrixTimes_GPU",
+             "matrixMatrixTimes_CPU", "matrixMatrixTimes_total"]
+
+
+@pytest.mark.parametrize("alg,extra", [("cg", CG_KEYS), ("cholesky", CHOL_KEYS)])
+def test_metrics_json_has_the_reference_keys(tmp_path, alg, extra):
+    import json
+
+    assert run("metrics", alg, tmp_path, 3)[0] == "ok"
+    text = (tmp_path / "metrics.json").read_text()
+    doc = json.loads(text, object_pairs_hook=lambda kv: kv)  # keep the key order
+    top = [k for k, _ in doc]
+    assert top == ["configuration", "runtime-metrics"]
+    conf_keys = [k for k, _ in doc[0][1]]
+    rt = doc[1][1]
+    assert conf_keys == CONFIG_KEYS
+    assert [k for k, _ in rt] == COMMON_KEYS + extra
+    vals = dict(rt)
+    # per-iteration vectors, one entry per iteration, std::to_string format
+    assert vals["iterationTimes"] == [0.5, 1.5, 2.5]
+    assert vals["blockCounts_GPU"] == [4, 4, 4] and vals["blockCounts_CPU"] == [0, 0, 0]
+    for k in extra:
+        assert len(vals[k]) == 3
+    assert "0.500000, 1.500000, 2.500000" in text
+    assert vals["totalTime"] == 12.25
