@@ -59,15 +59,16 @@ def build_host():
         return None
     out = os.path.join(LIBDIR, "libhs_host.so")
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
-          "-o", tmp] + srcs + ["-L" + LIBDIR, "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
+    # the CPU half of the heterogeneous solves runs on OpenMP
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
+          "-I" + os.path.join(ROOT, "include"), "-o", tmp] + srcs + ["-L" + LIBDIR, "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
     for src, name in ((os.path.join(HOST, "driver", "hs_driver.cpp"), "hs_driver"),
                       (os.path.join(HOST, "tests", "hs_host_tests.cpp"), "hs_host_tests")):
         if os.path.exists(src):
             exe = os.path.join(LIBDIR, name)
-            _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o", exe + ".tmp", src,
-                  "-L" + LIBDIR, "-lhs_host", "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
+            _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o",
+                  exe + ".tmp", src, "-L" + LIBDIR, "-lhs_host", "-lhs_amd", "-Wl,-rpath,$ORIGIN"])
             os.replace(exe + ".tmp", exe)
     return out
 

@@ -356,9 +356,10 @@ static int launch_partials_abs(hs_handle_t h, const double* A, const double* x, 
     return HS_OK;
 }
 
+// blocks = the first `blocks` stored blocks in packed order (0: all of them)
 static int launch_partials(hs_handle_t h, const double* A, const double* x, double* rowpart, double* colpart, int n,
-                           int b, const i64* stop, i64 it) {
-    const i64 blocks = (i64)n * (n + 1) / 2;
+                           int b, const i64* stop, i64 it, i64 blocks = 0) {
+    if (blocks <= 0) blocks = (i64)n * (n + 1) / 2;
     if (blocks > 0x7fffffffLL) {
         hs_set_error("SYMV: too many blocks");
         return HS_ERR_UNSUPPORTED;
@@ -438,6 +439,36 @@ extern "C" int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* 
     if ((rc = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, nullptr, 0))) return rc;
     const i64 rows = (i64)n * b;
     const i64 G = rows / FIN_ROWS;
+    symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(w.rowpart, w.colpart, n, b, y,
+                                                                                   nullptr, nullptr, nullptr, 0);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+// Rows [0, blockRows b) of y = A x: the block rows a GPU owns when the rows are
+// split between a CPU and a GPU (CG.cpp:43-46). Every block these rows touch
+// lies in block columns 0 .. blockRows-1 (blocks (I, J) with J <= I and, read
+// transposed, (J, I) with J > I), which is the packed prefix of the first
+// S = blockRows n - blockRows (blockRows - 1) / 2 blocks: the read-once pass
+// runs over that prefix only, the row sums over those rows only.
+extern "C" int hs_symv_rows_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b,
+                                int blockRows) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (!x) return -3;
+    if (!y) return -4;
+    if (N <= 0) return -5;
+    if (b <= 0) return -6;
+    const int n = (int)((N + b - 1) / b);
+    if (blockRows < 0 || blockRows > n) return -7;
+    if (blockRows == 0) return HS_OK;
+    if (!symv_fast_ok(b)) return hs_symv_block_f64(h, A, x, y, b, 0, 0, blockRows, n, n, 1);
+    SymvWork w;
+    int rc = symv_workspace(h, n, b, false, &w);
+    if (rc) return rc;
+    const i64 prefix = (i64)blockRows * n - (i64)blockRows * (blockRows - 1) / 2;
+    if ((rc = launch_partials(h, A, x, w.rowpart, w.colpart, n, b, nullptr, 0, prefix))) return rc;
+    const i64 G = (i64)blockRows * b / FIN_ROWS;
     symv_finish_kernel<0><<<dim3((unsigned)G), dim3(SYMV_THREADS), 0, h->stream>>>(w.rowpart, w.colpart, n, b, y,
                                                                                    nullptr, nullptr, nullptr, 0);
     HS_LAUNCH_CHECK();

@@ -107,6 +107,7 @@ int main(int argc, char** argv) {
         if (conf::checkResult) {
             // mean |A x - b| as UtilityFunctions::checkResult (UtilityFunctions.cpp:70-91), mat-vec on the GPU
             hs_handle_t h = gpuQueue.handle();
+            gpuQueue.reserve(A.N, A.blockSize);
             hs::DeviceArray<double> dA(A.matrixData.size()), dx(xres.size()), dy(xres.size());
             hs::check(hs_memcpy_htod(h, dA.data(), A.matrixData.data(), A.matrixData.size() * 8), "htod");
             hs::check(hs_memcpy_htod(h, dx.data(), xres.data(), xres.size() * 8), "htod");

@@ -1,7 +1,9 @@
 // CG, Cholesky and TriangularSystemSolver of the host API over the C ABI.
-// The reference splits rows between a CPU and a GPU queue by the load
-// balancer's share; this build runs the GPU share 1.0 only and rejects any
-// other share with std::invalid_argument.
+// At GPU share 1.0 the whole solve runs on the device; at a share below 1.0
+// the rows are split between the CPU and the GPU as the reference splits them
+// (hetero.cpp), and the triangular solves run on the CPU
+// (TriangularSystemSolver.cpp:18).
+#include "cpu_kernels.hpp"
 #include <chrono>
 #include <ctime>
 #include <filesystem>
@@ -26,13 +28,9 @@ std::string timeString() {
     return s.str();
 }
 
-void requireFullGPUShare(const LoadBalancer& lb, const char* who) {
-    if (lb.currentProportionGPU != 1.0) {
-        throw std::invalid_argument(std::string(who) +
-                                    ": this build runs at GPU share 1.0 only (heterogeneous CPU+GPU splitting is not "
-                                    "part of it); got share " +
-                                    std::to_string(lb.currentProportionGPU));
-    }
+void requireValidShare(double share, const char* who) {
+    if (!(share >= 0.0 && share <= 1.0))
+        throw std::invalid_argument(std::string(who) + ": GPU share must lie in [0, 1], got " + std::to_string(share));
 }
 
 // UtilityFunctions::writeResult (UtilityFunctions.cpp:19-33): one line, values
@@ -62,9 +60,14 @@ CG::CG(SymmetricMatrix& A, RightHandSide& b, hs::Queue& cpuQueue, hs::Queue& gpu
 }
 
 void CG::solveHeterogeneous() {
+    requireValidShare(loadBalancer->currentProportionGPU, "CG");
+    if (loadBalancer->currentProportionGPU < 1.0) {
+        solveSplit();
+        finish();
+        return;
+    }
     const auto start = clk::now();
     metricsTracker.startTracking();
-    requireFullGPUShare(*loadBalancer, "CG");
     hs_handle_t h = gpuQueue.handle();
     const std::size_t bytesA = A.matrixData.size() * sizeof(double);
     const std::size_t bytesV = b.rightHandSideData.size() * sizeof(double);
@@ -120,6 +123,11 @@ void CG::solveHeterogeneous() {
     hs::check(hs_memcpy_dtoh(h, x.data(), dx.data(), bytesV), "hs_memcpy_dtoh");
     metricsTracker.resultCopyTime = ms_since(t0);
     metricsTracker.totalTime = ms_since(start);
+    finish();
+}
+
+// CG.cpp:131-162: report, metrics.json under conf::outputPath, x_result.txt
+void CG::finish() {
     std::cout << "Total time: " << metricsTracker.totalTime << "ms (" << iterations << " iterations)" << std::endl;
     metricsTracker.endTracking();
     std::string dir = conf::outputPath + "/" + timeString();
@@ -137,9 +145,16 @@ Cholesky::~Cholesky() {
 }
 
 void Cholesky::solve_heterogeneous() {
+    requireValidShare(conf::initialProportionGPU, "Cholesky");
+    if (conf::initialProportionGPU < 1.0) {
+        solveSplit();
+        std::cout << "Total time:  " << metricsTracker.totalTime << "ms" << std::endl;
+        if (!conf::trackCholeskySolveStep) metricsTracker.endTracking();
+        if (conf::writeMatrix) MatrixParser::writeFullMatrix("./A_chol_result", A);
+        return;
+    }
     const auto start = clk::now();
     metricsTracker.startTracking();
-    requireFullGPUShare(*loadBalancer, "Cholesky");
     hs_handle_t h = gpuQueue.handle();
     const std::size_t bytes = A.matrixData.size() * sizeof(double);
     auto t0 = clk::now();
@@ -204,10 +219,15 @@ TriangularSystemSolver::TriangularSystemSolver(SymmetricMatrix& A, conf::fp_type
 
 double TriangularSystemSolver::solve() {
     const auto start = clk::now();
-    // the reference solves on the GPU only when the share is exactly 1
-    // (TriangularSystemSolver.cpp:18)
+    // the reference solves on the GPU only when the share is exactly 1, on the
+    // CPU otherwise (TriangularSystemSolver.cpp:18), with the factor in
+    // A.matrixData
     if (conf::initialProportionGPU != 1.0) {
-        throw std::invalid_argument("TriangularSystemSolver: this build solves on the GPU at share 1.0 only");
+        hs::cpu::tri_solve(A.matrixData.data(), b.rightHandSideData.data(), A.N, A.blockSize);
+        const double ms = ms_since(start);
+        std::cout << "Time to solve the triangular system: " << ms << "ms" << std::endl;
+        if (conf::writeResult) writeResult(".", b.rightHandSideData);
+        return ms;
     }
     if (!A_gpu) throw std::invalid_argument("TriangularSystemSolver: no device factor (run Cholesky first)");
     hs_handle_t h = gpuQueue.handle();

@@ -11,11 +11,15 @@
 //   writefull <matrix.txt> <bsz> <out>   MatrixParser::writeFullMatrix of the parsed matrix
 //   writeblocked <matrix.txt> <bsz> <out> MatrixParser::writeBlockedMatrix of it
 //   metrics <algorithm> <dir> <count>    MetricsTracker::writeJSON with <count> entries per vector
+//   solve <cg|cholesky> <share> <matrix.txt> <vector.txt> <bsz> <eps>
+//                                        the solver classes at a GPU share (0: CPU only,
+//                                        no device touched); prints iterations / info, then x
 // and, on a GPU:
 //   gemvgp <m> <n>                       matrixVectorGP on exact binary data
 //   kstar <train> <test> <N> <M>         generateTestKernelMatrix, all entries
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -118,6 +122,40 @@ int main(int argc, char** argv) {
             m.totalTime = 12.25;
             m.writeJSON(dir);
             std::cout << "ok" << std::endl;
+            return 0;
+        }
+        if (cmd == "solve") {
+            const std::string alg = argv[2];
+            const double share = std::stod(argv[3]);
+            std::string pathA = argv[4], pathB = argv[5];
+            conf::matrixBlockSize = std::stoul(argv[6]);
+            conf::workGroupSize = static_cast<int>(conf::matrixBlockSize);
+            conf::epsilon = std::stod(argv[7]);
+            conf::initialProportionGPU = share;
+            conf::algorithm = alg;
+            conf::outputPath = argc > 8 ? argv[8] : "./output";
+            std::unique_ptr<hs::Queue> gpu;
+            if (share > 0.0) gpu.reset(new hs::Queue(hs::gpu_selector_v));
+            hs::Queue& gq = gpu ? *gpu : host;
+            SymmetricMatrix A = MatrixParser::parseSymmetricMatrix(pathA, host);
+            RightHandSide b = MatrixParser::parseRightHandSide(pathB, host);
+            conf::N = A.N;
+            auto lb = std::make_shared<StaticLoadBalancer>(conf::updateInterval, share, A.blockCountXY);
+            std::vector<double> x;
+            if (alg == "cg") {
+                CG cg(A, b, host, gq, lb);
+                cg.solveHeterogeneous();
+                std::cout << "iterations " << cg.iterations << "\n";
+                x.assign(cg.x.begin(), cg.x.end());
+            } else {
+                Cholesky chol(A, host, gq, lb);
+                chol.solve_heterogeneous();
+                TriangularSystemSolver tss(A, chol.A_gpu, b, host, gq, lb);
+                tss.solve();
+                std::cout << "info " << chol.info << "\n";
+                x.assign(b.rightHandSideData.begin(), b.rightHandSideData.end());
+            }
+            for (double v : x) std::cout << v << "\n";
             return 0;
         }
         if (cmd == "gemvgp") {

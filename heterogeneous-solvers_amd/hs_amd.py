@@ -55,6 +55,7 @@ _SIGS = {
     "hs_syrk_gemm_trailing_f64": [_P, _P, _I, _I, _I],
     "hs_symv_block_f64": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
     "hs_symv_packed_f64": [_P, _P, _P, _P, _L, _I],
+    "hs_symv_rows_f64": [_P, _P, _P, _P, _L, _I, _I],
     "hs_symv_packed_f64_timed": [_P, _P, _P, _P, _L, _I, _I, C.POINTER(_D)],
     "hs_trsv_block_f64": [_P, _P, _P, _L, _I, _I, _I, _I],
     "hs_col_update_f64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I],

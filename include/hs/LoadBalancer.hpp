@@ -1,6 +1,6 @@
 // GPU-share policy interface (src/loadBalancer/LoadBalancer.hpp:10-28) and the
-// static policy (StaticLoadBalancer.cpp:6-14). The solvers of this build run
-// at share 1.0 only; a share below 1.0 is rejected by the solvers.
+// static policy (StaticLoadBalancer.cpp:6-14). The solvers run the whole
+// solve on the GPU at share 1.0 and split rows between CPU and GPU below it.
 #pragma once
 
 #include <stdexcept>

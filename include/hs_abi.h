@@ -135,6 +135,12 @@ int hs_symv_packed_f64(hs_handle_t h, const double* A, const double* x, double* 
  * two passes on the handle's stream; pass_ms[0] / pass_ms[1] receive the
  * average durations of the read-once block pass and of the row reduction.
  * Measurement entry; no reference counterpart. */
+/* Block rows [0, blockRows) of y = A x (y's other rows untouched): the GPU's
+ * share of a mat-vec split by rows between a CPU and a GPU
+ * (matrixVectorBlock_GPU over blockStart_i = 0, blockCount_i = blockRows, all
+ * columns; CG.cpp:342-398). Reads only the packed prefix of block columns
+ * 0 .. blockRows-1. */
+int hs_symv_rows_f64(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b, int blockRows);
 int hs_symv_packed_f64_timed(hs_handle_t h, const double* A, const double* x, double* y, int64_t N, int b, int reps,
                              double* pass_ms);
 

@@ -246,6 +246,22 @@ def parse_rhs_text(path):
     return np.array(parse_row_values(lines[1])[:N])
 
 
+def write_matrix_text(path, K):
+    """The reference's matrix format (MatrixParser.cpp:11-72): '# N', then
+    row i's entries 0..i separated by ';', 17 significant digits (exact)."""
+    N = K.shape[0]
+    with open(path, "w") as fh:
+        fh.write("# %d\n" % N)
+        for i in range(N):
+            fh.write(";".join("%.17g" % v for v in K[i, :i + 1]) + "\n")
+
+
+def write_rhs_text(path, y):
+    with open(path, "w") as fh:
+        fh.write("# %d\n" % len(y))
+        fh.write(";".join("%.17g" % v for v in y) + "\n")
+
+
 def fixture_matrix():
     return parse_symmetric_matrix_text(os.path.join(GOLDEN_DIR, "data", "testMatrixSymmetric20x20.txt"))
 

@@ -102,20 +102,7 @@ def test_driver_cg_on_gp_input_files(tmp_path, oracle):
     assert np.linalg.norm(x - xref) <= 1e-6 * np.linalg.norm(xref)
 
 
-def _write_matrix_text(path, K):
-    """The reference's matrix format (MatrixParser.cpp:11-72): '# N', then
-    row i's entries 0..i separated by ';', 17 significant digits (exact)."""
-    N = K.shape[0]
-    with open(path, "w") as fh:
-        fh.write("# %d\n" % N)
-        for i in range(N):
-            fh.write(";".join("%.17g" % v for v in K[i, :i + 1]) + "\n")
-
-
-def _write_rhs_text(path, y):
-    with open(path, "w") as fh:
-        fh.write("# %d\n" % len(y))
-        fh.write(";".join("%.17g" % v for v in y) + "\n")
+from hs_testlib import write_matrix_text as _write_matrix_text, write_rhs_text as _write_rhs_text  # noqa: E402
 
 
 @pytest.mark.parametrize("fma", [0, 1])
@@ -173,8 +160,72 @@ def test_driver_cholesky_on_gp_input_files(tmp_path, oracle):
     assert result(r, "relative_residual") < 1e-10
 
 
-def test_driver_rejects_partial_gpu_share(tmp_path):
-    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", 4, "--init_gpu_perc", 0.5,
+@pytest.mark.parametrize("share", [0.5, 0.25, 0.75])
+@pytest.mark.parametrize("bsz", [4, 6])
+def test_driver_split_share_on_reference_fixture(tmp_path, oracle, share, bsz):
+    """GPU share below 1.0 (--init_gpu_perc): the rows are split between the
+    CPU (the library's OpenMP kernels) and the GPU (the C ABI's per-op
+    kernels) as the reference splits them; the result is the fixture's
+    solution, and CG's count is the oracle's within 1 on this small system."""
+    ref = np.linalg.solve(fixture_matrix(), fixture_vector())
+    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", bsz, "--init_gpu_perc", share,
+              "--algorithm", "cholesky", "--write_result", "true", "--check_result", "true")
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(tmp_path / "x_result.txt")[:20]
+    assert np.linalg.norm(x - ref) <= 1e-12 * np.linalg.norm(ref)
+    rt = metrics(tmp_path)["runtime-metrics"]
+    assert any(c > 0 for c in rt["blockCounts_CPU"]) and any(g > 0 for g in rt["blockCounts_GPU"])
+    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", bsz, "--init_gpu_perc", share,
+              "--algorithm", "cg", "--eps", 1e-10, "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(tmp_path / "x_result.txt")[:20]
+    assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref)
+    n = -(-20 // bsz)
+    rhs = np.zeros(n * bsz)
+    rhs[:20] = fixture_vector()
+    from hs_testlib import dense_to_packed
+    _, itr, _ = oracle.cg(dense_to_packed(fixture_matrix(), bsz), rhs, 20, bsz, 1e-10)
+    assert abs(int(result(r, "iterations")) - itr) <= 1
+
+
+@pytest.mark.parametrize("share", [0.5, 0.25])
+def test_driver_split_share_on_gp_system(tmp_path, oracle, share):
+    """N = 2048, b = 128, sigma_n^2 = 1e-2 at a GPU share below 1.0: the
+    Cholesky solution within 1e-12 of the oracle's, CG converged with its
+    count within 5% of the oracle's (its dot products are the two sides'
+    partial sums, an order of their own)."""
+    from hs_testlib import packed_to_dense, synth_gp_numpy
+
+    N, b = 2048, 128
+    X, y = synth_gp_numpy(17, N, 8)
+    A = oracle.rbf(X, N, 8, b, 1.0, 1.0, 1e-2)
+    K = packed_to_dense(A, N, b)
+    _write_matrix_text(tmp_path / "A.txt", K)
+    _write_rhs_text(tmp_path / "b.txt", y)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    xref = oracle.tri_solve(oracle.cholesky(A, N, b), rhs, N, b)
+    r = drive(tmp_path, "--path_A", tmp_path / "A.txt", "--path_b", tmp_path / "b.txt", "--matrix_bsz", b,
+              "--init_gpu_perc", share, "--algorithm", "cholesky", "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(tmp_path / "x_result.txt")
+    rel = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    print("split share %.2f: cholesky rel diff vs oracle %.3e" % (share, rel), flush=True)
+    assert rel <= 1e-12
+    r = drive(tmp_path, "--path_A", tmp_path / "A.txt", "--path_b", tmp_path / "b.txt", "--matrix_bsz", b,
+              "--init_gpu_perc", share, "--algorithm", "cg", "--eps", 1e-10, "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    xc = np.loadtxt(tmp_path / "x_result.txt")
+    _, itr, _ = oracle.cg(A, rhs, N, b, 1e-10)
+    it = int(result(r, "iterations"))
+    print("split share %.2f: cg iterations %d, oracle %d" % (share, it, itr), flush=True)
+    assert abs(it - itr) <= max(1, itr // 20)
+    assert np.linalg.norm(K @ xc[:N] - y) <= 3e-10 * np.linalg.norm(y)
+
+
+def test_driver_rejects_invalid_share(tmp_path):
+    r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", 4, "--init_gpu_perc", 1.5,
               "--algorithm", "cg")
     assert r.returncode == 1
-    assert "share 1.0" in r.stderr
+    assert "Invalid GPU proportion" in r.stderr
