@@ -174,7 +174,11 @@ def test_driver_split_share_on_reference_fixture(tmp_path, oracle, share, bsz):
     x = np.loadtxt(tmp_path / "x_result.txt")[:20]
     assert np.linalg.norm(x - ref) <= 1e-12 * np.linalg.norm(ref)
     rt = metrics(tmp_path)["runtime-metrics"]
-    assert any(c > 0 for c in rt["blockCounts_CPU"]) and any(g > 0 for g in rt["blockCounts_GPU"])
+    n = -(-20 // bsz)
+    assert any(g > 0 for g in rt["blockCounts_GPU"])
+    # the GPU keeps at least conf::minBlockCountCholesky = 3 block rows: with
+    # 3 or fewer rows below the first diagonal block the CPU gets none
+    assert any(c > 0 for c in rt["blockCounts_CPU"]) == (n - 1 > 3)
     r = drive(tmp_path, "--path_A", MATRIX, "--path_b", VECTOR, "--matrix_bsz", bsz, "--init_gpu_perc", share,
               "--algorithm", "cg", "--eps", 1e-10, "--write_result", "true")
     assert r.returncode == 0, r.stderr
