@@ -352,21 +352,36 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
 }
 
 // Rest update from pre-split planes in 256 x 256 tiles (b % 256 == 0): 512
-// threads as 4 x 2 waves of 64 x 128, KT = 32 stages of the four planes
-// (80 KB each, double-buffered in 160 KB of dynamic LDS), the B fragments
-// taken in two halves of 64 columns to stay within 256 VGPRs. A quarter less
-// LDS read traffic and half the operand traffic per MFMA of the 128 tile.
-constexpr int T2 = 256, T2THREADS = 512, T2P = 40;
-constexpr int T2PLANE = T2 * T2P;                   // bf16 elements per plane
+// threads as 4 x 2 waves of 64 x 128, KT = 32 deep stages of the four planes
+// (A hi, A lo, B hi, B lo; 16 KB each), double-buffered in 128 KB of dynamic
+// LDS. Half the operand traffic per MFMA of the 128 tile.
+//  - Stages are filled by LDS DMA (global_load_lds_dwordx4: each lane's 16
+//    bytes land at the wave's LDS base + 16 * lane), so no VGPR staging and
+//    no LDS stores. Planes are unpadded, 64 bytes per row, with the 16-byte
+//    k-chunk q of row r in slot q ^ ((r >> 2) & 3): the fragment reads of 16
+//    lanes (rows r .. r+15, one chunk) then cover all 64 banks once. Wave w
+//    fills plane w / 2, rows (w & 1) * 128 .. +127, 16 rows per DMA, lane L
+//    taking row L / 4 and chunk (L & 3) ^ (L >> 4), which is slot L & 3.
+//  - A stage runs as two halves of 64 output columns. The second half's B
+//    fragments are read while the first half's MFMAs run; then one barrier
+//    (this wave's DMA of the next stage has landed, every wave's reads of the
+//    stage are done), the DMA of stage kt+2 into the buffer just consumed,
+//    and the next stage's A and first-half B fragments are read while the
+//    second half's MFMAs run. The barrier carries no memory fence: a fence
+//    would wait for the DMA in flight, which the explicit counters cover.
+constexpr int T2 = 256, T2THREADS = 512;
+constexpr int T2PLANE = T2 * 64;                    // bytes per plane and stage
 constexpr int T2STAGE = 4 * T2PLANE;
-constexpr size_t T2LDS = sizeof(unsigned short) * 2 * T2STAGE;
+constexpr size_t T2LDS = 2 * (size_t)T2STAGE;
+constexpr unsigned WAIT_VM0_LGKM0 = 0x0070;         // s_waitcnt vmcnt(0) lgkmcnt(0)
+constexpr unsigned WAIT_VM0 = 0x0F70;               // s_waitcnt vmcnt(0)
 
 __global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __restrict__ A, int b, int n, PairMap pm,
                                                                    const unsigned short* __restrict__ Wh,
                                                                    const unsigned short* __restrict__ Wl, int rs,
                                                                    int m) {
     constexpr int KT = 32;
-    extern __shared__ unsigned short lds2[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds2[];
     const int tpe = b / T2, tpb = tpe * tpe;
     int I, J, tile;
     if (!tile_of(pm, tpb, &I, &J, &tile)) return;
@@ -377,78 +392,96 @@ __global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __rest
     float* C = A + hs_block_id(I, J, n) * bb + (i64)tm * T2 * b + (i64)tn * T2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 128;
-    // staging: thread -> row tid / 2, 16 consecutive k from (tid & 1) * 16
-    const int lrow = tid >> 1, lk = (tid & 1) * 16;
-    const i64 wa = (i64)(I - rs) * bb + (i64)(tm * T2 + lrow) * b + lk;
-    const i64 wb = (i64)(J - rs) * bb + (i64)(tn * T2 + lrow) * b + lk;
-    u32x4 rah[2], ral[2], rbh[2], rbl[2];
+    const int plane = wave >> 1;
+    const unsigned short* src = (plane & 1) ? Wl : Wh;
+    const int rsel = plane < 2 ? I : J, tsel = plane < 2 ? tm : tn;
+    const int rr = (wave & 1) * 128 + (lane >> 2);
+    const int q = (lane & 3) ^ (lane >> 4);
+    const unsigned short* gp = src + (i64)(rsel - rs) * bb + (i64)(tsel * T2 + rr) * b + q * 8;
+    unsigned char* lb = lds2 + plane * T2PLANE + (wave & 1) * 8 * 1024;
+    const int nk = b / KT, nkt = nk * pm.np;
+    auto dma = [&](int kt, int buf) {
+        const i64 o = (i64)(kt / nk) * m * bb + (i64)(kt % nk) * KT;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            __builtin_amdgcn_global_load_lds(gp + o + (i64)j * 16 * b, lb + buf * T2STAGE + j * 1024, 16, 0, 0);
+    };
+    const int lr = lane & 15, lq = lane >> 4;
+    const int sw = (lq ^ (lr >> 2)) * 16;
+    bf8 ah[4], al[4], b0h[4], b0l[4], b1h[4], b1l[4];
+    auto readA = [&](const unsigned char* s) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int off = (wm0 + i * 16 + lr) * 64 + sw;
+            ah[i] = *reinterpret_cast<const bf8*>(s + off);
+            al[i] = *reinterpret_cast<const bf8*>(s + T2PLANE + off);
+        }
+    };
+    auto readB = [&](const unsigned char* s, int h, bf8 (&bh)[4], bf8 (&bl)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int off = (wn0 + (4 * h + j) * 16 + lr) * 64 + sw;
+            bh[j] = *reinterpret_cast<const bf8*>(s + 2 * T2PLANE + off);
+            bl[j] = *reinterpret_cast<const bf8*>(s + 3 * T2PLANE + off);
+        }
+    };
     f4 acc[4][8];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int nk = b / KT, nkt = nk * pm.np;
-    auto gload = [&](int kt) {
-        const i64 o = (i64)(kt / nk) * m * bb + (i64)(kt % nk) * KT;
+    auto half = [&](int h, const bf8 (&bh)[4], const bf8 (&bl)[4]) {
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            rah[v] = *reinterpret_cast<const u32x4*>(Wh + wa + o + 8 * v);
-            ral[v] = *reinterpret_cast<const u32x4*>(Wl + wa + o + 8 * v);
-            rbh[v] = *reinterpret_cast<const u32x4*>(Wh + wb + o + 8 * v);
-            rbl[v] = *reinterpret_cast<const u32x4*>(Wl + wb + o + 8 * v);
-        }
-    };
-    auto sstore = [&](int buf) {
-        u32x4* d = reinterpret_cast<u32x4*>(lds2 + buf * T2STAGE + lrow * T2P + lk);
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            d[v] = rah[v];
-            d[T2PLANE / 8 + v] = ral[v];
-            d[2 * T2PLANE / 8 + v] = rbh[v];
-            d[3 * T2PLANE / 8 + v] = rbl[v];
-        }
-    };
-    gload(0);
-    sstore(0);
-    if (nkt > 1) gload(1);
-    __syncthreads();
-    const int lr = lane & 15, lq = lane >> 4;
-    for (int kt = 0; kt < nkt; ++kt) {
-        const unsigned short* s = lds2 + (kt & 1) * T2STAGE;
-        bf8 ah[4], al[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int off = (wm0 + i * 16 + lr) * T2P + 8 * lq;
-            ah[i] = *reinterpret_cast<const bf8*>(s + off);
-            al[i] = *reinterpret_cast<const bf8*>(s + T2PLANE + off);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            bf8 bh[4], bl[4];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int off = (wn0 + (4 * h + j) * 16 + lr) * T2P + 8 * lq;
-                bh[j] = *reinterpret_cast<const bf8*>(s + 2 * T2PLANE + off);
-                bl[j] = *reinterpret_cast<const bf8*>(s + 3 * T2PLANE + off);
+                f4& c = acc[i][4 * h + j];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
             }
-            // the other buffer was last read in stage kt-1: fill it while
-            // this stage's MFMAs run
-            if (h == 0 && kt + 1 < nkt) {
-                sstore((kt + 1) & 1);
-                if (kt + 2 < nkt) gload(kt + 2);
-            }
+    };
+    dma(0, 0);
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    __syncthreads();
+    if (nkt > 1) dma(1, 1);
+    readA(lds2);
+    readB(lds2, 0, b0h, b0l);
+    for (int kt = 0; kt < nkt; ++kt) {
+        const unsigned char* s = lds2 + (kt & 1) * T2STAGE;
+        readB(s, 1, b1h, b1l);
+        half(0, b0h, b0l);
+        // 48 MFMAs, the 8 reads spread over the first 16
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    f4& c = acc[i][4 * h + j];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
-                }
+        for (int t = 0; t < 8; ++t) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
-        __syncthreads();
+        __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0_LGKM0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        // unconditional, so that the stage stays one basic block for the
+        // schedule above and below: past the last stage the DMA refetches
+        // stage 0 into the buffer just consumed, and the reads take the other
+        // buffer's contents, which are never used
+        dma(kt + 2 < nkt ? kt + 2 : 0, kt & 1);
+        half(1, b1h, b1l);
+        {
+            const unsigned char* s1 = lds2 + ((kt + 1) & 1) * T2STAGE;
+            readB(s1, 0, b0h, b0l);
+            readA(s1);
+        }
+        // 48 MFMAs with the 16 reads spread over the first 32
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 1);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 16, 1);
     }
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);  // no DMA into LDS outlives the work-group
     epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
