@@ -85,6 +85,12 @@ _SIGS = {
                             C.POINTER(_D)],
     "hs_symv_ref_order_f64": [_P, _P, _P, _P, _L, _I, _I],
     "hs_dot_ref_order_f64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
+    "hs_cholesky_ref_order_f64": [_P, _P, _L, _I, _I],
+    "hs_tri_solve_ref_order_f64": [_P, _P, _P, _L, _I, _I],
+    "hs_potrf_ref_order_f64": [_P, _P, _L, _I, _I, _I, _I],
+    "hs_trsm_ref_order_f64": [_P, _P, _I, _I, _I, _I, _I, _I],
+    "hs_syrk_ref_order_f64": [_P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "hs_gemm_ref_order_f64": [_P, _P, _I, _I, _I, _I, _I, _I, _I],
     "hs_rbf_assemble_f64": [_P, _P, _L, _I, _I, _D, _D, _D, _P],
     "hs_rbf_window_f64": [_P, _P, _L, _I, _P],
     "hs_to_f32_packed": [_P, _P, _P, _L, _I, _D],

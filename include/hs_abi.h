@@ -295,6 +295,35 @@ int hs_symv_ref_order_f64(hs_handle_t h, const double* A, const double* x, doubl
 int hs_dot_ref_order_f64(hs_handle_t h, const double* x, const double* y, int b, int blockStart, int blockCount,
                          int wg, int wgf, int fma, double* result_dev);
 
+/* The blocked Cholesky in the reference's own floating-point order, the
+ * parity mode of hs_cholesky_f64: per step k the kernels of
+ * Cholesky::solve_heterogeneous at share 1.0 (Cholesky.cpp:413-493), each
+ * element computed in the sequence of the reference kernel that writes it:
+ * POTRF as cholesky_optimizedGPU (MatrixOperations.cpp:124-190), TRSM as
+ * triangularSolve_optimizedGPU (MatrixMatrixOperations.cpp:50-107, the
+ * multiplication by 1.0 / l_ii), SYRK as symmetricMatrixMatrixDiagonal_
+ * optimizedGPU (:212-305, terms subtracted one by one), GEMM as
+ * matrixMatrixStep_optimizedGPU3 (:645-779, the reference's default
+ * gpuOptimizationLevel 3: terms summed from zero, then subtracted). fma = 1
+ * contracts each multiply-subtract / multiply-add statement, fma = 0 rounds
+ * every product and sum. b <= 1024, at most 257 block columns. Returns 0 or
+ * the first bad pivot (> 0). */
+int hs_cholesky_ref_order_f64(hs_handle_t h, double* A, int64_t N, int b, int fma);
+/* The forward and backward solves L L^T x = rhs in place on x in the same
+ * order (TriangularSystemSolver::solve, TriangularSystemSolver.cpp:15-105:
+ * triangularSolveBlockVector, MatrixVectorOperations.cpp:282-335, and
+ * matrixVectorColumnUpdate, :337-377), fma as above; entries >= N are left. */
+int hs_tri_solve_ref_order_f64(hs_handle_t h, const double* L, double* x, int64_t N, int b, int fma);
+/* Its four kernels on the reference's block coordinates (the arguments of
+ * the kernels they follow, cited above). */
+int hs_potrf_ref_order_f64(hs_handle_t h, double* A, int64_t N, int b, int blockID, int blockRow, int fma);
+int hs_trsm_ref_order_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart,
+                          int blockCount, int fma);
+int hs_syrk_ref_order_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart,
+                          int blockCount, int blockCountXY, int fma);
+int hs_gemm_ref_order_f64(hs_handle_t h, double* A, int b, int blockID, int blockRow, int blockStart,
+                          int blockCount, int blockCountXY, int fma);
+
 /* RBF kernel matrix into the packed layout on the device:
  * K_ij = sf2 * exp(-||x_i - x_j||^2 / (2 ell^2)) + sn2 [i == j],
  * X row-major N x d on the device. Padding entries are written as zero.

@@ -37,6 +37,17 @@
  *    solution, and, at fma = 0, a loop in the tests that composes the pinned
  *    or_symv, or_dot_tree, or_sub and or_scale_add in CG.cpp's step order
  *    (bit for bit);
+ *  - or_*_gpu_order (Cholesky): at fma = 0, POTRF, SYRK and GEMM bit for bit
+ *    against or_potrf, or_syrk and or_gemm, which the goldens pin; the TRSM
+ *    against the reference's triangularSolve_optimizedGPU goldens, and, on
+ *    inputs where it decides the rounding, against the multiplication by a
+ *    rounded reciprocal (so that a division fails); at fma = 1, single
+ *    statements on inputs where contraction changes the rounding, compared
+ *    with exactly rounded results (so that a missing or misplaced fma fails);
+ *    the whole factor against numpy's Cholesky within the rounding bound;
+ *  - or_tri_solve_gpu_order: bit for bit against or_tri_solve at fma = 0;
+ *    single statements where contraction changes the rounding at fma = 1;
+ *    scipy's cho_solve on the same factor within the rounding bound;
  *  - or_rbf / or_rbf_window: numpy brute force of the textbook RBF formula;
  *  - or_rbf_cross / or_window_features / or_gemv_gp: numpy brute force of the
  *    formula and of the window construction, K* = sf2 exactly where a test
@@ -538,6 +549,193 @@ i64 or_cg_gpu_order(const double* A, const double* rhs, double* x, i64 N, int b,
     free(d);
     free(q);
     return it;
+}
+
+/* ---------------------------------------------------------------------------
+ * The reference's share-1.0 GPU blocked Cholesky, in its arithmetic order.
+ * Cholesky::solve_heterogeneous at share 1.0 (Cholesky.cpp:413-493) runs per
+ * step k: MatrixOperations::cholesky_optimizedGPU (MatrixOperations.cpp:
+ * 124-190), MatrixMatrixOperations::triangularSolve_optimizedGPU
+ * (MatrixMatrixOperations.cpp:50-107) on blocks k+1 .. n-1,
+ * symmetricMatrixMatrixDiagonal_optimizedGPU (:212-305) on the diagonal
+ * blocks k+1 .. n-1 and, at the default gpuOptimizationLevel 3
+ * (Configuration.hpp:49), matrixMatrixStep_optimizedGPU3 (:645-779) on the
+ * off-diagonal blocks of rows k+2 .. n-1. Every block is written by one
+ * kernel per step, so only the order inside one element's computation
+ * matters:
+ *  - POTRF: right-looking by column; a_ij (i >= j) takes a_ij - a_ik a_jk for
+ *    k = 0 .. j-1 in turn, then a_ij / sqrt(a_jj) (i > j) with the diagonal's
+ *    own updates done, and a_jj = sqrt(a_jj). Rows >= N are not touched; the
+ *    upper triangle of the other rows is zeroed. This is or_potrf's order.
+ *  - TRSM, one row x of each block below: d_i = 1.0 / l_ii; v_i starts at the
+ *    row's value and takes v_i - l_ik x_k for k = 0 .. i-1 in turn, where
+ *    x_k = v_k d_k; the stored value is v_i d_i. The multiplication by the
+ *    reciprocal is where this order differs from the CPU kernel (or_trsm,
+ *    which divides).
+ *  - SYRK: v = a_ij (i >= j), then v - c_ik c_jk for k = 0 .. b-1 in turn
+ *    (or_syrk's order).
+ *  - GEMM: s = 0, s + b_ik c_jk for k = 0 .. b-1 in turn, then a_ij - s
+ *    (or_gemm's order).
+ * As for the CG, what the source leaves open is whether the compiler
+ * contracts those multiply-subtract / multiply-add statements; fma = 0 keeps
+ * every product rounded on its own (this file is built with
+ * -ffp-contract=off), fma = 1 contracts exactly the four `v - x y` /
+ * `s + x y` forms above with fma().
+ * ------------------------------------------------------------------------- */
+static inline double msub(double v, double x, double y, int fma_mode) {
+    return fma_mode ? fma(-x, y, v) : v - x * y;
+}
+
+void or_potrf_gpu_order(double* A, i64 N, int b, int blockID, int blockRow, int fma_mode) {
+    double* B = A + (i64)blockID * b * b;
+    for (int k = 0; k < b; ++k) {
+        const double sqrtDiag = sqrt(B[(i64)k * b + k]);
+        for (int i = k + 1; i < b; ++i)
+            if ((i64)blockRow * b + i < N) B[(i64)i * b + k] = B[(i64)i * b + k] / sqrtDiag;
+        for (int i = k + 1; i < b; ++i) {
+            if ((i64)blockRow * b + i >= N) continue;
+            for (int j = k + 1; j <= i; ++j)
+                B[(i64)i * b + j] = msub(B[(i64)i * b + j], B[(i64)i * b + k], B[(i64)j * b + k], fma_mode);
+        }
+        B[(i64)k * b + k] = sqrtDiag;
+    }
+    for (int i = 0; i < b; ++i) {
+        if ((i64)blockRow * b + i >= N) continue;
+        for (int j = i + 1; j < b; ++j) B[(i64)i * b + j] = 0;
+    }
+}
+
+void or_trsm_gpu_order(double* A, int b, int blockID, int blockRow, int blockStart, int blockCount, int fma_mode) {
+    const double* L = A + (i64)blockID * b * b;
+    double* dinv = (double*)malloc(sizeof(double) * b);
+    for (int i = 0; i < b; ++i) dinv[i] = 1.0 / L[(i64)i * b + i];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int g = 0; g < blockCount; ++g) {
+        for (int r = 0; r < b; ++r) {
+            double* row = A + ((i64)blockID + (blockStart - blockRow) + g) * b * b + (i64)r * b;
+            for (int i = 0; i < b; ++i) {
+                double v = row[i];
+                for (int k = 0; k < i; ++k) v = msub(v, L[(i64)i * b + k], row[k], fma_mode);
+                row[i] = v * dinv[i];
+            }
+        }
+    }
+    free(dinv);
+}
+
+void or_syrk_gpu_order(double* A, int b, int blockID, int blockRow, int blockStart, int blockCount, int n,
+                       int fma_mode) {
+#pragma omp parallel for schedule(static)
+    for (int g = 0; g < blockCount; ++g) {
+        const int diag = blockStart + g;
+        double* D = A + or_block_id(diag, diag, n) * b * b;
+        const double* C = A + ((i64)blockID + (blockStart - blockRow) + g) * b * b;
+        for (int i = 0; i < b; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double v = D[(i64)i * b + j];
+                for (int k = 0; k < b; ++k) v = msub(v, C[(i64)i * b + k], C[(i64)j * b + k], fma_mode);
+                D[(i64)i * b + j] = v;
+            }
+    }
+}
+
+void or_gemm_gpu_order(double* A, int b, int blockID, int blockRow, int blockStart, int blockCount, int n,
+                       int fma_mode) {
+    for (int I = blockStart; I < blockStart + blockCount; ++I) {
+        const double* Bm = A + ((i64)blockID + (I - blockRow)) * b * b;
+#pragma omp parallel for schedule(static)
+        for (int J = blockRow + 1; J < I; ++J) {
+            double* C = A + or_block_id(I, J, n) * b * b;
+            const double* Cm = A + ((i64)blockID + (J - blockRow)) * b * b;
+            for (int i = 0; i < b; ++i)
+                for (int j = 0; j < b; ++j) {
+                    double s = 0;
+                    for (int k = 0; k < b; ++k)
+                        s = fma_mode ? fma(Bm[(i64)i * b + k], Cm[(i64)j * b + k], s)
+                                     : s + Bm[(i64)i * b + k] * Cm[(i64)j * b + k];
+                    C[(i64)i * b + j] = C[(i64)i * b + j] - s;
+                }
+        }
+    }
+}
+
+void or_cholesky_gpu_order(double* A, i64 N, int b, int fma_mode) {
+    const int n = (int)((N + b - 1) / b);
+    for (int k = 0; k < n; ++k) {
+        const int blockID = (int)or_block_id(k, k, n);
+        or_potrf_gpu_order(A, N, b, blockID, k, fma_mode);
+        const int cnt = n - (k + 1);
+        if (cnt > 0) {
+            or_trsm_gpu_order(A, b, blockID, k, k + 1, cnt, fma_mode);
+            or_syrk_gpu_order(A, b, blockID, k, k + 1, cnt, n, fma_mode);
+        }
+        if (cnt > 1) or_gemm_gpu_order(A, b, blockID, k, k + 2, cnt - 1, n, fma_mode);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * The triangular solves after the factor, in the order of the reference's
+ * kernels with the contraction choice of the factor above:
+ * TriangularSystemSolver::solve (TriangularSystemSolver.cpp:15-105) runs the
+ * same two kernels on the GPU queue at share 1.0 as on the CPU queue:
+ * triangularSolveBlockVector (MatrixVectorOperations.cpp:282-335; b_k =
+ * b_k / a_kk, then b_i - a_ik b_k) and matrixVectorColumnUpdate (:337-377;
+ * s = 0, s + a_ik b_k, then b_i - s). At fma = 0 this is or_tri_solve.
+ * ------------------------------------------------------------------------- */
+static void trsv_block_fma(const double* A, double* x, i64 N, int b, int blockRow, int blockID, int transposed,
+                           int fma_mode) {
+    const double* L = A + (i64)blockID * b * b;
+    double* xb = x + (i64)blockRow * b;
+    for (int s = 0; s < b; ++s) {
+        const int k = transposed ? b - (s + 1) : s;
+        const double bk = xb[k] / L[(i64)k * b + k];
+        if ((i64)blockRow * b + k >= N) continue;
+        xb[k] = bk;
+        for (int i = 0; i < b; ++i) {
+            const int cond = transposed ? (i < k) : (i > k);
+            if (!cond) continue;
+            xb[i] = msub(xb[i], transposed ? L[(i64)k * b + i] : L[(i64)i * b + k], bk, fma_mode);
+        }
+    }
+}
+
+static void col_update_fma(const double* A, double* x, int b, int blockStart, int blockCount, int blockRow,
+                           int blockID, int n, int transposed, int fma_mode) {
+    const i64 total = (i64)n * (n + 1) / 2;
+#pragma omp parallel for schedule(static)
+    for (int g = 0; g < blockCount; ++g) {
+        i64 id;
+        if (!transposed) id = (i64)blockID + (blockStart - blockRow + g);
+        else {
+            const i64 m = n - g - blockStart;
+            id = total - m * (m + 1) / 2 + blockRow - g - blockStart;
+        }
+        const double* M = A + id * b * b;
+        const double* x0 = x + (i64)blockRow * b;
+        double* xi = x + (i64)(blockStart + g) * b;
+        for (int li = 0; li < b; ++li) {
+            double sum = 0.0;
+            for (int k = 0; k < b; ++k) {
+                const double a = transposed ? M[(i64)k * b + li] : M[(i64)li * b + k];
+                sum = fma_mode ? fma(a, x0[k], sum) : sum + a * x0[k];
+            }
+            xi[li] = xi[li] - sum;
+        }
+    }
+}
+
+void or_tri_solve_gpu_order(const double* L, double* x, i64 N, int b, int fma_mode) {
+    const int n = (int)((N + b - 1) / b);
+    for (int j = 0; j < n; ++j) {
+        const int id = (int)or_block_id(j, j, n);
+        trsv_block_fma(L, x, N, b, j, id, 0, fma_mode);
+        if (j < n - 1) col_update_fma(L, x, b, j + 1, n - (j + 1), j, id, n, 0, fma_mode);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        const int id = (int)or_block_id(j, j, n);
+        trsv_block_fma(L, x, N, b, j, id, 1, fma_mode);
+        if (j > 0) col_update_fma(L, x, b, 0, j, j, id, n, 1, fma_mode);
+    }
 }
 
 /* ---------------------------------------------------------------------------

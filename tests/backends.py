@@ -57,6 +57,23 @@ class OracleBackend:
         self.a[res][0] = self.o.or_dot_tree(dptr(self.a[x]), dptr(self.a[y]), b, bs, bc, wg, 512, self.fma)
 
 
+class OracleGpuOrderBackend(OracleBackend):
+    """The oracle with the factor kernels in the order of the reference's
+    share-1.0 GPU kernels (or_*_gpu_order)."""
+
+    def potrf(self, A, N, b, bid, brow):
+        self.o.or_potrf_gpu_order(dptr(self.a[A]), N, b, bid, brow, self.fma)
+
+    def trsm(self, A, b, bid, brow, bs, bc):
+        self.o.or_trsm_gpu_order(dptr(self.a[A]), b, bid, brow, bs, bc, self.fma)
+
+    def syrk(self, A, b, bid, brow, bs, bc, n):
+        self.o.or_syrk_gpu_order(dptr(self.a[A]), b, bid, brow, bs, bc, n, self.fma)
+
+    def gemm(self, A, b, bid, brow, bs, bc, n):
+        self.o.or_gemm_gpu_order(dptr(self.a[A]), b, bid, brow, bs, bc, n, self.fma)
+
+
 class HipBackend:
     """Runs every op through libhs_amd.so on the GPU."""
 
@@ -116,3 +133,20 @@ class HipBackend:
     def dot_tree(self, x, y, res, b, bs, bc, wg):
         self._c("hs_dot_ref_order_f64", self.buf[x].p, self.buf[y].p, b, bs, bc, wg, 512, self.fma,
                 self.buf[res].p)
+
+
+class HipRefOrderBackend(HipBackend):
+    """HipBackend with the factor kernels of the reference-order Cholesky
+    (hs_*_ref_order_f64)."""
+
+    def potrf(self, A, N, b, bid, brow):
+        self._c("hs_potrf_ref_order_f64", self.buf[A].p, N, b, bid, brow, self.fma)
+
+    def trsm(self, A, b, bid, brow, bs, bc):
+        self._c("hs_trsm_ref_order_f64", self.buf[A].p, b, bid, brow, bs, bc, self.fma)
+
+    def syrk(self, A, b, bid, brow, bs, bc, n):
+        self._c("hs_syrk_ref_order_f64", self.buf[A].p, b, bid, brow, bs, bc, n, self.fma)
+
+    def gemm(self, A, b, bid, brow, bs, bc, n):
+        self._c("hs_gemm_ref_order_f64", self.buf[A].p, b, bid, brow, bs, bc, n, self.fma)

@@ -53,6 +53,12 @@ def kernel_cases():
     return [t for t in GOLDENS["tests"] if t["calls"]]
 
 
+def factor_cases():
+    """Cases whose every call is a blocked-Cholesky kernel (POTRF, TRSM,
+    SYRK, GEMM)."""
+    return [t for t in kernel_cases() if all(OPS.get(c[0]) in ("potrf", "trsm", "syrk", "gemm") for c in t["calls"])]
+
+
 def parser_cases():
     return [t for t in GOLDENS["tests"] if not t["calls"]]
 

@@ -70,6 +70,12 @@ class Oracle:
             "or_dot_tree": ([_P, _P, _I, _I, _I, _I, _I, _I], _D),
             "or_symv_ordered": ([_P, _P, _P, _I, _I, _I], None),
             "or_cg_gpu_order": ([_P, _P, _P, _L, _I, _D, _L, _I, _I, _I, C.POINTER(_D)], _L),
+            "or_potrf_gpu_order": ([_P, _L, _I, _I, _I, _I], None),
+            "or_trsm_gpu_order": ([_P, _I, _I, _I, _I, _I, _I], None),
+            "or_syrk_gpu_order": ([_P, _I, _I, _I, _I, _I, _I, _I], None),
+            "or_gemm_gpu_order": ([_P, _I, _I, _I, _I, _I, _I, _I], None),
+            "or_cholesky_gpu_order": ([_P, _L, _I, _I], None),
+            "or_tri_solve_gpu_order": ([_P, _P, _L, _I, _I], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -85,6 +91,17 @@ class Oracle:
         A = A.copy()
         self.L.or_cholesky(dptr(A), N, b)
         return A
+
+    def cholesky_gpu_order(self, A, N, b, fma=0):
+        """Blocked Cholesky in the order of the reference's share-1.0 GPU path."""
+        A = A.copy()
+        self.L.or_cholesky_gpu_order(dptr(A), N, b, fma)
+        return A
+
+    def tri_solve_gpu_order(self, L_, x, N, b, fma=0):
+        x = x.copy()
+        self.L.or_tri_solve_gpu_order(dptr(L_), dptr(x), N, b, fma)
+        return x
 
     def tri_solve(self, L_, x, N, b):
         x = x.copy()
