@@ -169,9 +169,18 @@ void Cholesky::solve_heterogeneous() {
 
     const std::size_t n = static_cast<std::size_t>(A.blockCountXY);
     std::vector<double> potrf(n), trsm(n), update(n);
-    const int rc = hs_cholesky_f64_trace(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize, potrf.data(), trsm.data(),
-                                         update.data());
-    hs::check(rc, "hs_cholesky_f64");
+    int rc;
+    if (conf::referenceOrder) {
+        // the parity mode: the reference's kernels' arithmetic order, no
+        // per-column record (the vectors below hold 0)
+        rc = hs_cholesky_ref_order_f64(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize,
+                                       conf::referenceOrderFMA ? 1 : 0);
+        hs::check(rc, "hs_cholesky_ref_order_f64");
+    } else {
+        rc = hs_cholesky_f64_trace(h, A_gpu, static_cast<int64_t>(A.N), A.blockSize, potrf.data(), trsm.data(),
+                                   update.data());
+        hs::check(rc, "hs_cholesky_f64");
+    }
     info = rc;
     if (info > 0) std::cerr << "warning: non-positive pivot at row " << info << " (matrix not SPD)" << std::endl;
     // The reference's per-column record (Cholesky.cpp:261-359). SYRK and GEMM
@@ -235,7 +244,12 @@ double TriangularSystemSolver::solve() {
     const std::size_t bytes = b.rightHandSideData.size() * sizeof(double);
     hs::DeviceArray<double> db(b.rightHandSideData.size());
     hs::check(hs_memcpy_htod(h, db.data(), b.rightHandSideData.data(), bytes), "hs_memcpy_htod");
-    hs::check(hs_tri_solve_f64(h, A_gpu, db.data(), static_cast<int64_t>(A.N), A.blockSize), "hs_tri_solve_f64");
+    if (conf::referenceOrder)
+        hs::check(hs_tri_solve_ref_order_f64(h, A_gpu, db.data(), static_cast<int64_t>(A.N), A.blockSize,
+                                             conf::referenceOrderFMA ? 1 : 0),
+                  "hs_tri_solve_ref_order_f64");
+    else
+        hs::check(hs_tri_solve_f64(h, A_gpu, db.data(), static_cast<int64_t>(A.N), A.blockSize), "hs_tri_solve_f64");
     hs::check(hs_memcpy_dtoh(h, b.rightHandSideData.data(), db.data(), bytes), "hs_memcpy_dtoh");
     const double ms = ms_since(start);
     std::cout << "Time to solve the triangular system: " << ms << "ms" << std::endl;

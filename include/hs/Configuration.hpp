@@ -37,10 +37,11 @@ struct Configuration {
     bool printVerbose = false;
     bool checkResult = false;
     bool trackCholeskySolveStep = true;
-    // Keys of this build (no reference counterpart): CG in the reference's own
-    // floating-point order (its GPU kernels' summation order, the
-    // work-group sizes above), so that iteration counts compare with the
-    // reference's GPU build; and whether that order contracts multiply-adds.
+    // Keys of this build (no reference counterpart): CG, Cholesky and the
+    // triangular solves in the reference's own floating-point order (its GPU
+    // kernels' summation order, the work-group sizes above), so that results
+    // compare with the reference's GPU build bit for bit; and whether that
+    // order contracts multiply-adds.
     bool referenceOrder = false;
     bool referenceOrderFMA = true;
 };

@@ -106,6 +106,32 @@ from hs_testlib import write_matrix_text as _write_matrix_text, write_rhs_text a
 
 
 @pytest.mark.parametrize("fma", [0, 1])
+def test_driver_cholesky_reference_order_solution(tmp_path, oracle, fma):
+    """--reference_order with --algorithm cholesky: the host Cholesky and
+    triangular solves run the reference's GPU arithmetic order; on the
+    oracle's own matrix the solution is the oracle's restatement of that
+    order, equal to the resolution x_result.txt prints (20 decimals)."""
+    from hs_testlib import packed_to_dense, synth_gp_numpy
+
+    N, b = 1000, 64
+    X, y = synth_gp_numpy(4, N, 8)
+    A = oracle.rbf(X, N, 8, b, 1.0, 1.0, 1e-2)
+    _write_matrix_text(tmp_path / "A.txt", packed_to_dense(A, N, b))
+    _write_rhs_text(tmp_path / "b.txt", y)
+    r = drive(tmp_path, "--path_A", tmp_path / "A.txt", "--path_b", tmp_path / "b.txt", "--matrix_bsz", b,
+              "--init_gpu_perc", 1, "--algorithm", "cholesky", "--reference_order", "true",
+              "--fma", "true" if fma else "false", "--write_result", "true")
+    assert r.returncode == 0, r.stderr
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    xref = oracle.tri_solve_gpu_order(oracle.cholesky_gpu_order(A, N, b, fma), rhs, N, b, fma)
+    check_x_result(tmp_path / "x_result.txt", n * b)
+    x = np.array([float(t) for t in open(tmp_path / "x_result.txt").read().split()])
+    assert np.max(np.abs(x - xref)) <= 1e-20
+
+
+@pytest.mark.parametrize("fma", [0, 1])
 def test_driver_cg_reference_order_iterations(tmp_path, oracle, fma):
     """--reference_order: the host CG runs the reference's GPU arithmetic
     order; on the oracle's own matrix (written in the reference's text format,
