@@ -126,6 +126,28 @@ def cholesky_workload(hs, dev, syn, args):
     res = rhs - Kx
     rel_res = float(np.linalg.norm(res) / np.linalg.norm(rhs))
     bwd = float(np.max(np.abs(res)) / (knorm_inf * np.max(np.abs(x)) + np.max(np.abs(rhs))))
+    # untimed: the same system through the reference-order factor and solves
+    # (the parity mode, FMA contracted), against the fast path's solution
+    ref_order = None
+    if not args.no_ref_order:
+        dR = dev.alloc(plen * 8)
+        hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, sf2, sn2, dR.p)
+        dxr = dev.upload(rhs)
+        t_ref = hs.Timer(dev)
+        t_ref.start()
+        info_r = hs.call("hs_cholesky_ref_order_f64", dev.h, dR.p, N, b, 1)
+        hs.call("hs_tri_solve_ref_order_f64", dev.h, dR.p, dxr.p, N, b, 1)
+        t_ref.stop()
+        ref_ms = t_ref.ms()
+        dR.free()
+        xr = dev.download(dxr, n * b)
+        hs.call("hs_symv_packed_f64", dev.h, dA.p, dxr.p, dr.p, N, b)
+        rr = rhs - dev.download(dr, n * b)
+        ref_order = {"ms": ref_ms, "info": info_r, "fma": 1,
+                     "rel_residual": float(np.linalg.norm(rr) / np.linalg.norm(rhs)),
+                     "rel_diff_vs_fast": float(np.linalg.norm(xr - x) / np.linalg.norm(xr))}
+        log("  reference-order factor + solves: %.0f ms, relative residual %.3e, fast path differs by %.3e"
+            % (ref_ms, ref_order["rel_residual"], ref_order["rel_diff_vs_fast"]))
     dA.free()
     ph = np.array(phases).mean(axis=0)
     pt = np.array(parts).mean(axis=0)
@@ -147,6 +169,7 @@ def cholesky_workload(hs, dev, syn, args):
         "update": {"launches": nl, "flops_per_launch": upd_flops / nl, "avg_launch_ms": float(ph[2]) / nl,
                    "tflops": achieved},
         "rel_residual": rel_res, "backward_error_inf": bwd,
+        "reference_order": ref_order,
         "_x": x[:N],  # the FP64 solution, for the mixed variants' forward difference (not printed)
     }
 
@@ -539,6 +562,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=14336)
     ap.add_argument("--cpu-cg-iters", type=int, default=30)
     ap.add_argument("--no-gpr", action="store_true")
+    ap.add_argument("--no-ref-order", action="store_true",
+                    help="skip the untimed reference-order (parity mode) solve of the Cholesky workload")
     ap.add_argument("--only", choices=["cg", "symv", "adjoint", "gpr", "mixed"],
                     help="run only this side workload and print its record (for profiling; not a bench line)")
     ap.add_argument("--copy-calibration", action="store_true",

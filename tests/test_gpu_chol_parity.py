@@ -118,3 +118,50 @@ def test_ref_order_argument_errors(hs, dev):
     assert lib.hs_potrf_ref_order_f64(dev.h, d.p, 64, 2048, 0, 0, 0) == hs.HS_ERR_UNSUPPORTED
     # SYRK over block rows past the matrix (n = 3, rows 1 .. 5)
     assert lib.hs_syrk_ref_order_f64(dev.h, d.p, 64, 0, 0, 1, 5, 3, 0) == -4
+
+
+def test_ref_order_at_the_full_workload_size(hs, dev):
+    """The N = 65536 workload (d = 16, b = 256, l = 2, sn2 = 1e-3, seed 3), past
+    the oracle's reach: the reference-order factor and solves against the fast
+    path through size-independent properties. Both solutions leave a relative
+    residual at the FP64 path's level, and they differ by no more than the
+    conditioning allows (kappa <= ||K||_inf / sn2)."""
+    import ctypes as C
+
+    N, d, b, ell, sn2 = 65536, 16, 256, 2.0, 1e-3
+    X, y = synth_gp_numpy(3, N, d)
+    n = -(-N // b)
+    rhs = np.zeros(n * b)
+    rhs[:N] = y
+    packed = n * (n + 1) // 2 * b * b
+    dX = dev.upload(X.reshape(-1))
+    dK, dR, dF = dev.zeros(packed), dev.zeros(packed), dev.zeros(packed)
+    for buf in (dK, dR, dF):
+        hs.call("hs_rbf_assemble_f64", dev.h, dX.p, N, d, b, ell, 1.0, sn2, buf.p)
+    dev.sync()
+    t0 = time.time()
+    assert hs.call("hs_cholesky_ref_order_f64", dev.h, dR.p, N, b, 1) == 0
+    dxr = dev.upload(rhs)
+    hs.call("hs_tri_solve_ref_order_f64", dev.h, dR.p, dxr.p, N, b, 1)
+    dev.sync()
+    t1 = time.time()
+    assert hs.call("hs_cholesky_f64", dev.h, dF.p, N, b) == 0
+    dxf = dev.upload(rhs)
+    hs.call("hs_tri_solve_f64", dev.h, dF.p, dxf.p, N, b)
+    dev.sync()
+    t2 = time.time()
+    xr, xf = dev.download(dxr, n * b), dev.download(dxf, n * b)
+
+    def rel_residual(x):
+        dx, dy = dev.upload(x), dev.zeros(n * b)
+        hs.call("hs_symv_packed_f64", dev.h, dK.p, dx.p, dy.p, N, b)
+        return np.linalg.norm(dev.download(dy, n * b)[:N] - y) / np.linalg.norm(y)
+
+    rr, rf = rel_residual(xr), rel_residual(xf)
+    diff = np.linalg.norm(xr - xf) / np.linalg.norm(xr)
+    say("N=65536 reference order: %.1f s; fast: %.2f s; relative residual %.3e / %.3e; solutions differ by %.3e"
+        % (t1 - t0, t2 - t1, rr, rf, diff))
+    assert rr <= 1e-10 and rf <= 1e-10
+    # kappa <= ||K||_inf / sn2 <= N sf2 / sn2: two backward-stable solves differ by O(kappa u)
+    assert diff <= (N / sn2) * np.finfo(float).eps
+    assert np.all(xr[N:] == 0.0) and np.all(xf[N:] == 0.0)
