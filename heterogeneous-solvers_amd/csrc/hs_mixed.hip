@@ -571,6 +571,25 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     return HS_OK;
 }
 
+// The fused trailing update of one right-looking step on FP32 storage, in
+// either variant (the mixed factor's per-op entry points).
+static int trailing_mixed_step(hs_handle_t h, float* A, int b, int k, int blockCountXY, int variant) {
+    if (!h) return -1;
+    if (!A) return -2;
+    if (b <= 0) return -3;
+    if (k < 0 || k >= blockCountXY) return -4;
+    return hs_launch_trailing_mixed(h, h->stream, A, b, k, k + 1, blockCountXY - k - 1, blockCountXY, 3, variant, 1,
+                                    nullptr, 0);
+}
+
+extern "C" int hs_syrk_gemm_trailing_f32(hs_handle_t h, float* A, int b, int k, int blockCountXY) {
+    return trailing_mixed_step(h, A, b, k, blockCountXY, HS_MIXED_FP32);
+}
+
+extern "C" int hs_syrk_gemm_trailing_bf16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY) {
+    return trailing_mixed_step(h, A, b, k, blockCountXY, HS_MIXED_BF16X3);
+}
+
 extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, double shift) {
     if (!h) return -1;
     if (!K) return -2;

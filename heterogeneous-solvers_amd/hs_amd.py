@@ -95,6 +95,8 @@ _SIGS = {
     "hs_rbf_window_f64": [_P, _P, _L, _I, _P],
     "hs_to_f32_packed": [_P, _P, _P, _L, _I, _D],
     "hs_cholesky_mixed": [_P, _P, _P, _L, _I, _I, _D],
+    "hs_syrk_gemm_trailing_f32": [_P, _P, _I, _I, _I],
+    "hs_syrk_gemm_trailing_bf16x3": [_P, _P, _I, _I, _I],
     "hs_cholesky_mixed_timed": [_P, _P, _P, _L, _I, _I, _D, C.POINTER(_D), C.POINTER(_L)],
     "hs_refine_solve": [_P, _P, _P, _P, _P, _L, _I, _D, _I, _I, _P],
     "hs_refine_solve_to": [_P, _P, _P, _P, _P, _L, _I, _D, _D, _I, _I, _P],

@@ -348,6 +348,15 @@ int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* 
  * FP32 storage, trailing SYRK/GEMM per `variant`, FP32 accumulation.
  * matrix_bsz must be 128 or 256. Returns 0 or the first bad pivot (>0). */
 int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift);
+/* The fused trailing update of right-looking step k on an FP32-stored packed
+ * factor (SYRK on every diagonal block, GEMM on every off-diagonal block of
+ * the trailing triangle, k < J <= I < n): FP32 MFMA, or BF16 hi+lo with three
+ * products, accumulated in FP32; b % 128 == 0. The mixed-precision siblings of
+ * hs_syrk_gemm_trailing_f64 (symmetricMatrixMatrixDiagonal_optimizedGPU +
+ * matrixMatrixStep_optimizedGPU3 of the reference's FP32 build,
+ * MatrixMatrixOperations.cpp:212-304, 645-778). */
+int hs_syrk_gemm_trailing_f32(hs_handle_t h, float* A, int b, int k, int blockCountXY);
+int hs_syrk_gemm_trailing_bf16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY);
 /* Same with per-phase kernel times (as hs_cholesky_f64_timed). */
 int hs_cholesky_mixed_timed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift,
                             double* phase_ms, int64_t* launches);

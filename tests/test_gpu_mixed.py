@@ -149,3 +149,38 @@ def test_mixed_reports_bad_pivot(hs, dev):
     A = dense_to_packed(K, b)
     dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
     assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, 1, 0.0) == 301
+
+
+@pytest.mark.parametrize("entry,tol", [("hs_syrk_gemm_trailing_f32", 2e-6), ("hs_syrk_gemm_trailing_bf16x3", 2e-5)])
+@pytest.mark.parametrize("N,b,k", [(1024, 128, 2), (1536, 256, 0), (1280, 256, 3)])
+def test_trailing_step_entries(hs, dev, entry, tol, N, b, k):
+    """One fused trailing-update step on FP32 storage: A_IJ -= L_Ik L_Jk^T for
+    k < J <= I < n (lower triangle of the diagonal blocks), against the same
+    update in FP64 on the stored FP32 values; every other entry untouched."""
+    n = -(-N // b)
+    rng = np.random.default_rng(N + b + k)
+    A = rng.uniform(-1, 1, packed_len(N, b)).astype(np.float32)
+    dA = dev.upload(A)
+    assert hs.call(entry, dev.h, dA.p, b, k, n) == 0
+    got = dev.download(dA, len(A), dtype=np.float32).astype(np.float64)
+    ref = A.astype(np.float64)
+
+    def blk(I, J):
+        s = (J * n - J * (J - 1) // 2 + (I - J)) * b * b
+        return slice(s, s + b * b)
+
+    for J in range(k + 1, n):
+        LJ = ref[blk(J, k)].reshape(b, b)
+        for I in range(J, n):
+            LI = ref[blk(I, k)].reshape(b, b)
+            upd = ref[blk(I, J)].reshape(b, b) - LI @ LJ.T
+            if I == J:
+                upd = np.where(np.tril(np.ones((b, b))) > 0, upd, ref[blk(I, J)].reshape(b, b))
+            ref[blk(I, J)] = upd.reshape(-1)
+    # |sum over b products of entries <= 1|: the error bar scales with b
+    assert np.max(np.abs(got - ref)) <= tol * b
+    changed = np.zeros(len(A), dtype=bool)
+    for J in range(k + 1, n):
+        for I in range(J, n):
+            changed[blk(I, J)] = True
+    assert np.array_equal(got[~changed], A.astype(np.float64)[~changed])
