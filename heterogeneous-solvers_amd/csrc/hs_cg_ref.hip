@@ -17,14 +17,18 @@
 // This is the parity mode. The fast CG (hs_cg.hip) reads each stored block
 // once and sums in its own fixed order.
 //
-// Layout of the mat-vec: one work-group per block row I, thread t owns row
-// I*b + t and keeps its running sum in a register for the whole pass. The
-// pass walks the block row as a stream of CH-column chunks. Row-wise blocks
-// are loaded coalesced (CH consecutive doubles of each row per 16 lanes),
-// transposed into a double-buffered LDS tile and read back as column t;
-// transposed blocks are read directly, lane t taking element t of each
-// stored row. The next chunk's loads are issued before the current chunk's
-// multiply-adds.
+// Layout of the mat-vec: one work-group per band of up to 128 rows of a
+// block row I (two bands per block row at b = 256, so that every CU gets
+// one), thread t owns one row and keeps its running sum in a register for
+// the whole pass. The pass walks the block row as a stream of CH-column
+// chunks. Row-wise blocks are loaded coalesced (CH consecutive doubles of
+// each row per 16 lanes), transposed into a double-buffered LDS tile and
+// read back as column t; transposed blocks are read directly, lane t taking
+// element t of each stored row. The next chunk's loads are issued before
+// the current chunk's multiply-adds. When b is a multiple of the chunk and
+// of the band, the loads carry no bounds tests (FULL): guarded, each load
+// became its own branch and the loads of a chunk no longer overlapped
+// (2.5x slower at b = 128).
 #include <algorithm>
 #include <climits>
 
@@ -50,40 +54,46 @@ __device__ __forceinline__ double add_nc(double a, double b) {
     return a + b;
 }
 
-template <int CH, bool FMA>
+template <int CH, bool FMA, bool FULL>
 __global__ __launch_bounds__(256) void symv_ordered_kernel(const double* __restrict__ A, const double* __restrict__ x,
-                                                           double* __restrict__ y, int n, int b,
+                                                           double* __restrict__ y, int n, int b, int RB,
                                                            const i64* __restrict__ stop, i64 it) {
     if (stop && it > *stop) return;
-    extern __shared__ double tile[];  // [2][CH][b + 1]: the odd pitch spreads a chunk row over the banks
-    const int I = blockIdx.x;
+    extern __shared__ double tile[];  // [2][CH][RB + 1]: the odd pitch spreads a chunk row over the banks
+    const int bands = (b + RB - 1) / RB;
+    const int I = blockIdx.x / bands;
+    const int rb0 = (blockIdx.x % bands) * RB;  // the band's first row in the block row
+    const int rows = min(RB, b - rb0);
     const int t = threadIdx.x;
     const int T = blockDim.x;
     const int cpb = (b + CH - 1) / CH;  // chunks per block
     const int K = n * cpb;
     const i64 bb = (i64)b * b;
-    const int P = b + 1;
+    const int P = RB + 1;
     double R[CH];
     double acc = 0.0;
 
-    // loads of chunk k into R: row-wise chunks as a b x CH sub-matrix in
+    // loads of chunk k into R: row-wise chunks as a rows x CH sub-matrix in
     // row-major element order e = r * CH + c (lane e % T), transposed chunks
-    // as rows c0..c0+CH-1 of the stored block, lane t taking column t
+    // as rows c0..c0+CH-1 of the stored block, lane t taking column rb0 + t
     auto load = [&](int k) {
         const int J = k / cpb, c0 = (k % cpb) * CH;
         if (J <= I) {
-            const double* blk = A + hs_block_id(I, J, n) * bb;
+            const double* blk = A + hs_block_id(I, J, n) * bb + (i64)rb0 * b;
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const int e = t + u * T;
                 const int r = e / CH, c = e % CH;
-                R[u] = (r < b && c0 + c < b) ? __builtin_nontemporal_load(blk + (i64)r * b + c0 + c) : 0.0;
+                if constexpr (FULL) R[u] = __builtin_nontemporal_load(blk + (i64)r * b + c0 + c);
+                else R[u] = (r < rows && c0 + c < b) ? __builtin_nontemporal_load(blk + (i64)r * b + c0 + c) : 0.0;
             }
         } else {
-            const double* blk = A + hs_block_id(J, I, n) * bb;
+            const double* blk = A + hs_block_id(J, I, n) * bb + rb0;
 #pragma unroll
-            for (int u = 0; u < CH; ++u)
-                R[u] = (t < b && c0 + u < b) ? __builtin_nontemporal_load(blk + (i64)(c0 + u) * b + t) : 0.0;
+            for (int u = 0; u < CH; ++u) {
+                if constexpr (FULL) R[u] = __builtin_nontemporal_load(blk + (i64)(c0 + u) * b + t);
+                else R[u] = (t < rows && c0 + u < b) ? __builtin_nontemporal_load(blk + (i64)(c0 + u) * b + t) : 0.0;
+            }
         }
     };
 
@@ -98,14 +108,14 @@ __global__ __launch_bounds__(256) void symv_ordered_kernel(const double* __restr
             for (int u = 0; u < CH; ++u) {
                 const int e = t + u * T;
                 const int r = e / CH, c = e % CH;
-                if (r < b) tl[c * P + r] = R[u];
+                if (FULL || r < rows) tl[c * P + r] = R[u];
             }
         }
         __syncthreads();
         double V[CH];
         if (rowwise) {
 #pragma unroll
-            for (int c = 0; c < CH; ++c) V[c] = (t < b) ? tl[c * P + t] : 0.0;
+            for (int c = 0; c < CH; ++c) V[c] = (FULL || t < rows) ? tl[c * P + t] : 0.0;
         } else {
 #pragma unroll
             for (int c = 0; c < CH; ++c) V[c] = R[c];
@@ -114,9 +124,9 @@ __global__ __launch_bounds__(256) void symv_ordered_kernel(const double* __restr
         const double* xs = x + (i64)J * b + c0;
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-            if (c < cw) acc = madd<FMA>(acc, V[c], xs[c]);
+            if (FULL || c < cw) acc = madd<FMA>(acc, V[c], xs[c]);
     }
-    if (t < b) y[(i64)I * b + t] = acc;
+    if (FULL || t < rows) y[(i64)I * b + rb0 + t] = acc;
 }
 
 // scalarProduct: one work-group of wg items per 2 wg elements of x y over
@@ -238,29 +248,35 @@ bool pow2_in(int v, int hi) { return v >= 1 && v <= hi && (v & (v - 1)) == 0; }
 // ---------------------------------------------------------------------------
 bool hs_symv_ordered_ok(int b) { return b >= 1 && b <= 256; }
 
-template <int CH, bool FMA>
+template <int CH, bool FMA, bool FULL>
 static int launch_ordered(hs_handle_t h, const double* A, const double* x, double* y, int n, int b, const i64* stop,
                           i64 it) {
-    // two CH x (b + 1) tiles: 66 KB at b = 128 (CH = 32) and b = 256 (CH = 16)
-    const size_t lds = sizeof(double) * 2 * CH * (b + 1);
+    const int RB = std::min(b, 128);             // rows per work-group
+    const int bands = (b + RB - 1) / RB;
+    // two CH x (RB + 1) tiles: 66 KB at CH = 32
+    const size_t lds = sizeof(double) * 2 * CH * (RB + 1);
     static bool raised = false;
     if (!raised) {
-        HS_HIP(hipFuncSetAttribute((const void*)symv_ordered_kernel<CH, FMA>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 2 * CH * 257)));
+        HS_HIP(hipFuncSetAttribute((const void*)symv_ordered_kernel<CH, FMA, FULL>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 2 * CH * 129)));
         raised = true;
     }
-    const int T = std::max(64, (b + 63) / 64 * 64);
-    symv_ordered_kernel<CH, FMA><<<dim3(n), dim3(T), lds, h->stream>>>(A, x, y, n, b, stop, it);
+    const int T = std::max(64, (RB + 63) / 64 * 64);
+    symv_ordered_kernel<CH, FMA, FULL><<<dim3((unsigned)(n * bands)), dim3(T), lds, h->stream>>>(A, x, y, n, b, RB,
+                                                                                              stop, it);
     HS_LAUNCH_CHECK();
     return HS_OK;
 }
 
 int hs_launch_symv_ordered(hs_handle_t h, const double* A, const double* x, double* y, int n, int b, int fma,
                            const i64* stop, i64 it) {
-    if (b <= 128) return fma ? launch_ordered<32, true>(h, A, x, y, n, b, stop, it)
-                             : launch_ordered<32, false>(h, A, x, y, n, b, stop, it);
-    return fma ? launch_ordered<16, true>(h, A, x, y, n, b, stop, it)
-               : launch_ordered<16, false>(h, A, x, y, n, b, stop, it);
+    // FULL: every band has 64k rows and every chunk 32 columns
+    const bool full = b % 32 == 0 && (b <= 128 ? b % 64 == 0 : b % 128 == 0);
+    if (full)
+        return fma ? launch_ordered<32, true, true>(h, A, x, y, n, b, stop, it)
+                   : launch_ordered<32, false, true>(h, A, x, y, n, b, stop, it);
+    return fma ? launch_ordered<32, true, false>(h, A, x, y, n, b, stop, it)
+               : launch_ordered<32, false, false>(h, A, x, y, n, b, stop, it);
 }
 
 // partial count of scalarProduct over len elements with wg items
