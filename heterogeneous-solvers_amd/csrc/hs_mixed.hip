@@ -14,6 +14,7 @@
 //                       ~2^-17 relative).
 // The steps follow the same one-panel look-ahead schedule as the FP64 path.
 #include <algorithm>
+#include <cstdlib>
 
 #include "hs_factor.h"
 #include "hs_internal.h"
@@ -485,6 +486,107 @@ __global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __rest
     epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
+// FP32 rest update in 256 x 256 tiles (b % 256 == 0), the FP32 sibling of
+// trailing_bf16x3_ps256: 512 threads as 4 x 2 waves of 64 x 128, KT = 16
+// deep stages of the two operand planes (A rows of block (I, k+p), B rows of
+// block (J, k+p); 16 KB each, 64-byte rows with the same chunk swizzle)
+// filled by LDS DMA straight from the FP32 panels, double-buffered in 64 KB.
+// A lane's 16-byte fragment read holds four consecutive k; MFMA step s of a
+// stage contracts k = 4 g + s over the lane groups g, the same for A and B.
+// One work-group fills a CU (two waves per SIMD), so a panel kernel finds a
+// free CU whenever one of them finishes.
+constexpr int F2PLANE = T2 * 64;                    // bytes per plane and stage
+constexpr int F2STAGE = 2 * F2PLANE;
+constexpr size_t F2LDS = 2 * (size_t)F2STAGE;
+
+__global__ __launch_bounds__(T2THREADS) void trailing_f32_ps256(float* __restrict__ A, int b, int n, PairMap pm) {
+    constexpr int KT = 16;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds3[];
+    const int tpe = b / T2, tpb = tpe * tpe;
+    int I, J, tile;
+    if (!tile_of(pm, tpb, &I, &J, &tile)) return;
+    const int tm = tile / tpe, tn = tile % tpe;
+    const bool diag = (I == J);
+    if (diag && tm < tn) return;
+    const i64 bb = (i64)b * b;
+    float* C = A + hs_block_id(I, J, n) * bb + (i64)tm * T2 * b + (i64)tn * T2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 128;
+    // staging: waves 0-3 fill the A plane, 4-7 the B plane, 64 rows each, 16
+    // rows per DMA; lane L takes row L / 4 of the 16 and k-chunk
+    // (L & 3) ^ (L >> 4), which lands in slot L & 3
+    const int plane = wave >> 2;
+    const int rr = (wave & 3) * 64 + (lane >> 2);
+    const int q = (lane & 3) ^ (lane >> 4);
+    const float* gp = A + hs_block_id(plane ? J : I, pm.k, n) * bb + (i64)((plane ? tn : tm) * T2 + rr) * b + q * 4;
+    unsigned char* lb = lds3 + plane * F2PLANE + (wave & 3) * 4 * 1024;
+    const int nk = b / KT, nkt = nk * pm.np;
+    // np panels: block (I, k+p) lies p (n-k-1) - p (p-1)/2 blocks after block (I, k)
+    auto koff = [&](int kt) -> i64 {
+        const i64 pp = kt / nk;
+        return (pp * (n - pm.k - 1) - pp * (pp - 1) / 2) * bb + (i64)(kt % nk) * KT;
+    };
+    auto dma = [&](int kt, int buf) {
+        const i64 o = koff(kt);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            __builtin_amdgcn_global_load_lds(gp + o + (i64)j * 16 * b, lb + buf * F2STAGE + j * 1024, 16, 0, 0);
+    };
+    const int lr = lane & 15, lq = lane >> 4;
+    const int sw = (lq ^ (lr >> 2)) * 16;
+    f4 a[4], b0[4], b1[4];
+    auto readA = [&](const unsigned char* st) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const f4*>(st + (wm0 + i * 16 + lr) * 64 + sw);
+    };
+    auto readB = [&](const unsigned char* st, int h, f4 (&bv)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bv[j] = *reinterpret_cast<const f4*>(st + F2PLANE + (wn0 + (4 * h + j) * 16 + lr) * 64 + sw);
+    };
+    f4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    auto half = [&](int h, const f4 (&bv)[4]) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][4 * h + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], bv[j][s], acc[i][4 * h + j], 0, 0, 0);
+    };
+    dma(0, 0);
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    __syncthreads();
+    if (nkt > 1) dma(1, 1);
+    readA(lds3);
+    readB(lds3, 0, b0);
+    for (int kt = 0; kt < nkt; ++kt) {
+        const unsigned char* st = lds3 + (kt & 1) * F2STAGE;
+        readB(st, 1, b1);
+        half(0, b0);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0_LGKM0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        // unconditional, to keep the stage one basic block: past the last
+        // stage the DMA refetches stage 0 into the buffer just consumed and
+        // the reads take the other buffer's contents, never used
+        dma(kt + 2 < nkt ? kt + 2 : 0, kt & 1);
+        half(1, b1);
+        {
+            const unsigned char* s1 = lds3 + ((kt + 1) & 1) * F2STAGE;
+            readB(s1, 0, b0);
+            readA(s1);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);  // no DMA into LDS outlives the work-group
+    epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+}
+
 // The bf16 hi / lo planes of np panels (block columns k .. k+np-1) for the
 // m block rows from rs: W[(p m + i) b^2 + e] for block (rs + i, k + p),
 // hi = rn(x), lo = rn(x - hi), eight values per thread.
@@ -519,6 +621,13 @@ __global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __r
 // planes (BF16 hi+lo only): the np panels k .. k+np-1 as bf16 hi / lo planes
 // for the block rows from r0 (split_panels_bf16_kernel layout, m = n - r0);
 // null for an update that splits the FP32 values it loads.
+// FP32 updates of several panels, and whole-triangle steps, in 256 x 256
+// tiles (HS_F32_TILE256=0 keeps the 128 x 128 kernel)
+static const bool g_f32_ps256 = [] {
+    const char* e = std::getenv("HS_F32_TILE256");
+    return !(e && e[0] == '0');
+}();
+
 int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
                              int n, int mode, int variant, int np, const unsigned short* planes, int r0) {
     if (rowCount <= 0) return HS_OK;
@@ -563,6 +672,17 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
         } else {
             trailing_mfma_bf16x3<TILE, 0, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
         }
+    } else if (b % T2 == 0 && (np > 1 || mode == 3) && g_f32_ps256) {
+        static bool attr = false;
+        if (!attr) {
+            HS_HIP(hipFuncSetAttribute((const void*)trailing_f32_ps256, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)F2LDS));
+            attr = true;
+        }
+        PairMap pm2;
+        const i64 grid2 = hs_pair_map(k, rowStart, rowCount, n, mode, np, (i64)(b / T2) * (b / T2), &pm2);
+        if (grid2 > 0)
+            trailing_f32_ps256<<<dim3((unsigned)grid2), dim3(T2THREADS), F2LDS, stream>>>(A, b, n, pm2);
     } else {
         if (mode >= 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f32<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
