@@ -517,10 +517,10 @@ def kernel_fractions(ch, mixed, cg, adj):
         "achieved": sol, "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": sol / HBM_PEAK_GBS}
     if mixed:
         f32 = mixed["fp32"]["update_tflops"]
-        out["trailing_mfma_f32 (FP32 update)"] = {"achieved": f32, "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
-                                                  "frac": f32 / FP32_PEAK_TFLOPS}
+        out["trailing_f32_ps256 + trailing_mfma_f32 (FP32 update)"] = {
+            "achieved": f32, "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "frac": f32 / FP32_PEAK_TFLOPS}
         b3 = mixed["bf16x3"]["update_tflops"]
-        out["trailing_bf16x3 (BF16 hi+lo update)"] = {
+        out["trailing_bf16x3_ps256 + trailing_mfma_bf16x3 (BF16 hi+lo update)"] = {
             "achieved": b3, "achieved_three_products": 3 * b3, "unit": "TFLOP/s", "peak": BF16_PEAK_TFLOPS,
             "frac": b3 / BF16_PEAK_TFLOPS, "frac_three_products": 3 * b3 / BF16_PEAK_TFLOPS}
     if cg:
