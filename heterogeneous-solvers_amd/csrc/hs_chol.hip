@@ -11,6 +11,8 @@
 // groups read 128 contiguous bytes, in KT = 16 deep stages with the next
 // stage's global loads in flight during the MFMAs of the current one.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "hs_factor.h"
@@ -173,8 +175,16 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f64(double* __restrict_
 // Dispatch: MFMA kernel when b is a multiple of 64 (the reference's optimised
 // GEMM has the same constraint, MatrixMatrixOperations.cpp:657-659), else the
 // element-wise kernel.
+// A half-CU launch asks for HALF_CU_LDS bytes of dynamic LDS it does not use:
+// 67.5 + 16 KB per work-group keeps a second one off the CU (160 KB), while a
+// column update (67.5 KB) or TRSM still fits beside it. A POTRF launched
+// beside half-CU updates asks for HALF_CU_POTRF_LDS, more than the free half,
+// so that it starts only on a CU of its own.
+constexpr int HALF_CU_LDS = 16 * 1024;
+constexpr size_t HALF_CU_POTRF_LDS = 80 * 1024;
+
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
-                          int mode, int np) {
+                          int mode, int np, bool halfCu) {
     if (rowCount <= 0) return HS_OK;
     if (b % 64 != 0) {
         if (mode > 3) {
@@ -194,8 +204,19 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
     }
     const dim3 g((unsigned)grid), t(THREADS);
     if (tile == 128) {
-        if (mode >= 5) trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
-        else trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+        if (mode >= 5) {
+            trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
+        } else if (halfCu) {
+            static bool attr = false;
+            if (!attr) {
+                HS_HIP(hipFuncSetAttribute((const void*)trailing_mfma_f64<128, 0>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, HALF_CU_LDS));
+                attr = true;
+            }
+            trailing_mfma_f64<128, 0><<<g, t, HALF_CU_LDS, stream>>>(A, b, n, pm);
+        } else {
+            trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
+        }
     } else {
         if (mode >= 5) trailing_mfma_f64<64, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f64<64, 0><<<g, t, 0, stream>>>(A, b, n, pm);
@@ -208,17 +229,51 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
     return hs_launch_trailing_on(h, h->stream, A, b, k, rowStart, rowCount, n, mode, 1);
 }
 
+// Rest updates of at most this many block rows run half-CU (one work-group
+// per CU). A step's rest update takes P b^3 m^2 flops for m rows; its panel
+// chain P (POTRF + TRSM + one-panel column update), about 1.1 us per unit of
+// b for each panel when those kernels run on a CU of their own, and up to
+// three times that beside full-CU update work-groups. Half-CU costs the
+// update a fifth of its rate; measured best (GP NLL at N = 16384, b = 128:
+// NLL 39.2 -> 37.6 ms) with the bound b^3 m^2 / 60 TFLOP/s < 2.5 * 1.1 us * b,
+// i.e. m <= 99 at b = 128 and m <= 50 at b = 256 (no change at N = 65536).
+// HS_HALF_CU_ROWS sets another bound (0 turns it off).
+static int hs_half_cu_rows(int b) {
+    static int env = -2;
+    if (env == -2) {
+        const char* e = std::getenv("HS_HALF_CU_ROWS");
+        env = e ? std::max(0, std::atoi(e)) : -1;
+    }
+    if (env >= 0) return env;
+    if (b % 128 != 0) return 0;
+    const double m2 = 60e12 * 2.5 * 1.1e-6 * b / ((double)b * b * b);
+    return (int)std::sqrt(m2);
+}
+
 // Look-ahead FP64 factorisation (schedule in hs_factor.h).
 static int cholesky_lookahead(hs_handle_t h, double* A, i64 N, int b, PhaseTimer& pt) {
     const int n = (int)((N + b - 1) / b);
     int rc = hs_reset_info(h);
     if (rc) return rc;
-    auto potrf = [&](int k) { return hs_factor_potrf<double>(h, A, N, b, hs_block_id(k, k, n), k); };
+    const int halfCuRows = hs_half_cu_rows(b);
+    const int P = hs_schedule_panels(n);
+    // The chain of panels k0 .. k0+P-1 (k0 > 0) runs beside the rest update
+    // of step k0 - P, over m = n - k0 - P rows. When that update is half-CU,
+    // its POTRFs ask for more LDS than the free half holds (80 KB beside
+    // 83.5 KB), so they start only on a CU with no update work-group: beside
+    // one, the POTRF's dependent FP64 arithmetic waits behind the update's
+    // FP64 MFMAs and takes three to four times as long.
+    auto potrf = [&](int k) {
+        const int k0 = k / P * P, m = n - k0 - P;
+        const size_t minLds = (k0 > 0 && m >= 1 && m <= halfCuRows) ? HALF_CU_POTRF_LDS : 0;
+        return hs_factor_potrf<double>(h, A, N, b, hs_block_id(k, k, n), k, minLds);
+    };
     auto trsm = [&](int k, int start, int count) {
         return hs_factor_trsm<double>(h, A, b, hs_block_id(k, k, n), k, start, count);
     };
     auto trail = [&](hipStream_t s, int k, int mode, int rowStart, int rowCount, int np) {
-        return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode, np);
+        return hs_launch_trailing_on(h, s, A, b, k, rowStart, rowCount, n, mode, np,
+                                     mode == 4 && rowCount <= halfCuRows);
     };
     return hs_lookahead_factor(h, n, pt, potrf, trsm, trail);
 }

@@ -139,15 +139,19 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
 // same on an explicit stream with np panels k .. k+np-1; modes 4 (the
 // triangle from row k + 2 np), 5 (block column k+1) and 6 (block columns
 // k+np, k+np+1) split a step for the look-ahead driver
+// (halfCu: one 128-tile work-group per CU, leaving the other half of every CU
+// to the panel chain's kernels)
 int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, int k, int rowStart, int rowCount, int n,
-                          int mode, int np);
+                          int mode, int np, bool halfCu = false);
 // panel kernels for b % 32 == 0 (POTRF) / b % 16 == 0 (TRSM), b <= 256;
 // W = the 16x16 diagonal sub-block inverses of the factor (POTRF writes them
 // when W is non-null; a TRSM given W == nullptr computes them into trsm_w)
 bool hs_potrf_fast_ok(int b);
 bool hs_trsm_fast_ok(int b);
+// (minLds: the launch asks for at least this much LDS, so that it only starts
+// on a CU without other resident work)
 template <typename ST>
-int hs_launch_potrf_fast(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, double* W);
+int hs_launch_potrf_fast(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, double* W, size_t minLds = 0);
 template <typename ST>
 int hs_launch_trsm_fast(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount,
                         const double* W);
@@ -155,7 +159,7 @@ int hs_launch_trsm_fast(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, 
 // leaves the sub-block inverses for the TRSM of the same column), else the
 // element-wise kernels (FP64 storage only)
 template <typename ST>
-int hs_factor_potrf(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow);
+int hs_factor_potrf(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, size_t minLds = 0);
 template <typename ST>
 int hs_factor_trsm(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount);
 // fast blocked triangular solves (b in {64, 128, 256}); factor in FP64 or FP32

@@ -305,12 +305,12 @@ bool hs_potrf_fast_ok(int b) { return b % 32 == 0 && b >= 32 && b <= 256; }
 bool hs_trsm_fast_ok(int b) { return b % 16 == 0 && b >= 16 && b <= 256; }
 
 template <typename ST>
-int hs_launch_potrf_fast(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, double* W) {
+int hs_launch_potrf_fast(hs_handle_t h, ST* A, i64 N, int b, i64 blockID, int blockRow, double* W, size_t minLds) {
     if (!hs_potrf_fast_ok(b)) {
         hs_set_error("panel POTRF needs b % 32 == 0 and b <= 256");
         return HS_ERR_UNSUPPORTED;
     }
-    const size_t lds = ((size_t)b * PP + 2 * PW) * sizeof(double);
+    const size_t lds = std::max(((size_t)b * PP + 2 * PW) * sizeof(double), minLds);
     if (lds > 64 * 1024)
         HS_HIP(hipFuncSetAttribute((const void*)potrf_rows_kernel<ST>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
@@ -357,13 +357,13 @@ int hs_launch_trsm_fast(hs_handle_t h, ST* A, int b, i64 blockID, int blockRow, 
 }
 
 template <>
-int hs_factor_potrf<double>(hs_handle_t h, double* A, i64 N, int b, i64 blockID, int blockRow) {
-    if (hs_potrf_fast_ok(b)) return hs_launch_potrf_fast<double>(h, A, N, b, blockID, blockRow, h->trsm_w);
+int hs_factor_potrf<double>(hs_handle_t h, double* A, i64 N, int b, i64 blockID, int blockRow, size_t minLds) {
+    if (hs_potrf_fast_ok(b)) return hs_launch_potrf_fast<double>(h, A, N, b, blockID, blockRow, h->trsm_w, minLds);
     return hs_launch_potrf(h, A, N, b, blockID, blockRow);
 }
 template <>
-int hs_factor_potrf<float>(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow) {
-    return hs_launch_potrf_fast<float>(h, A, N, b, blockID, blockRow, h->trsm_w);
+int hs_factor_potrf<float>(hs_handle_t h, float* A, i64 N, int b, i64 blockID, int blockRow, size_t minLds) {
+    return hs_launch_potrf_fast<float>(h, A, N, b, blockID, blockRow, h->trsm_w, minLds);
 }
 template <>
 int hs_factor_trsm<double>(hs_handle_t h, double* A, int b, i64 blockID, int blockRow, int blockStart, int blockCount) {
@@ -378,7 +378,7 @@ int hs_factor_trsm<float>(hs_handle_t h, float* A, int b, i64 blockID, int block
                                       hs_potrf_fast_ok(b) ? h->trsm_w : nullptr);
 }
 
-template int hs_launch_potrf_fast<double>(hs_handle_t, double*, i64, int, i64, int, double*);
-template int hs_launch_potrf_fast<float>(hs_handle_t, float*, i64, int, i64, int, double*);
+template int hs_launch_potrf_fast<double>(hs_handle_t, double*, i64, int, i64, int, double*, size_t);
+template int hs_launch_potrf_fast<float>(hs_handle_t, float*, i64, int, i64, int, double*, size_t);
 template int hs_launch_trsm_fast<double>(hs_handle_t, double*, int, i64, int, int, int, const double*);
 template int hs_launch_trsm_fast<float>(hs_handle_t, float*, int, i64, int, int, int, const double*);
