@@ -30,6 +30,7 @@
 // result does not depend on scheduling.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "hs_internal.h"
 
@@ -211,19 +212,40 @@ constexpr int SKT = 16;               // k stage depth
 constexpr int SLDP = ST + 4;          // LDS pitch (doubles)
 constexpr int SSTAGE = SKT * SLDP;
 
+// One launch covers the panel blocks I = I0 .. I0 + nrows - 1 and the block
+// columns J0 .. J1 - 1 in chunks; chunk c of row I goes to partial slot
+// slot0 + c: W[(slot0 + c) m + I - k - 1]. A launch may carry a second such
+// range (nrows2 > 0) after the first one's nrows x nch work-groups.
+struct SymmPart {
+    int I0, nrows, J0, J1, chunk, slot0;
+    int nch = 0, I02 = 0, nrows2 = 0, J02 = 0, J12 = 0, chunk2 = 1, slot02 = 0;
+};
+
 __global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restrict__ Ab, const double* __restrict__ L,
                                                           double* __restrict__ W, int b, int n, int k, int m,
-                                                          int chunk, int tilesN) {
+                                                          SymmPart sp, int tilesN) {
     __shared__ double As[2 * SSTAGE];
     __shared__ double Bs[2 * SSTAGE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tilesPer = tilesN * tilesN;
     const int tile = blockIdx.x % tilesPer;
-    const int rest = blockIdx.x / tilesPer;
-    const int ip = rest % m;
-    const int c = rest / m;
-    const int I = k + 1 + ip;
-    const int Ja = k + 1 + c * chunk, Jb = std::min(n, Ja + chunk);
+    int rest = blockIdx.x / tilesPer;
+    int I, c, Ja, Jb, slot;
+    if (sp.nrows2 > 0 && rest >= sp.nrows * sp.nch) {
+        rest -= sp.nrows * sp.nch;
+        I = sp.I02 + rest % sp.nrows2;
+        c = rest / sp.nrows2;
+        Ja = sp.J02 + c * sp.chunk2;
+        Jb = std::min(sp.J12, Ja + sp.chunk2);
+        slot = sp.slot02 + c;
+    } else {
+        I = sp.I0 + rest % sp.nrows;
+        c = rest / sp.nrows;
+        Ja = sp.J0 + c * sp.chunk;
+        Jb = std::min(sp.J1, Ja + sp.chunk);
+        slot = sp.slot0 + c;
+    }
+    const int ip = I - k - 1;
     const int tm = tile / tilesN, tn = tile % tilesN;
     const i64 bb = (i64)b * b;
     const int nkb = b / SKT;                  // stages per J block
@@ -323,7 +345,7 @@ __global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restri
         __syncthreads();
     }
     // output: register v of lane l is row l/16 + 4v, column l%16 of its tile
-    double* Wt = W + ((i64)c * m + ip) * bb + (i64)(tm * ST) * b + tn * ST;
+    double* Wt = W + ((i64)slot * m + ip) * bb + (i64)(tm * ST) * b + tn * ST;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -499,21 +521,40 @@ inline int symm_chunk(int m, int tiles) {
     return (m + nch - 1) / nch;
 }
 
+// Partial slots of reverse step k on the 128-tile path (m = n - k - 1 panel
+// blocks): slot 0 holds the block column J = k + 1 for every row, slots
+// 1 .. bulk_chunks(m) the chunks of J = k + 2 .. n - 1.
+inline int bulk_chunk(int m, int stiles) { return symm_chunk(std::max(1, m - 1), stiles); }
+inline int bulk_chunks(int m, int stiles) {
+    if (m < 2) return 0;
+    const int ch = bulk_chunk(m, stiles);
+    return (m - 1 + ch - 1) / ch;
+}
+
 // Workspace layout (doubles) of one gradient evaluation.
 struct AdjWork {
-    i64 dinv, v, z, zs, nll, parts, grad, wp, wq, wg, t1, t2, t3, total;
+    i64 dinv, v, z, zs, nll, parts, grad, wp, wp_half, wq, wg, t1, t2, t3, total;
 };
 
 AdjWork adj_layout(int n, int b) {
     const i64 bb = (i64)b * b, nb = (i64)n * b;
     const int tiles = (b % ST == 0) ? (b / ST) * (b / ST) : (b / AT) * (b / AT);
+    // partials of the trailing-adjoint product: one buffer of nch x m blocks
+    // (64-tile path), or two alternating buffers of (1 + bulk chunks) x m
+    // blocks (128-tile path)
     i64 wp = 0;
     for (int k = 0; k + 1 < n; ++k) {
         const int m = n - k - 1;
-        const int ch = symm_chunk(m, tiles);
-        const i64 nch = (m + ch - 1) / ch;
-        wp = std::max(wp, nch * m * bb);
+        if (b % ST == 0) {
+            wp = std::max(wp, (1 + (i64)bulk_chunks(m, tiles)) * m * bb);
+        } else {
+            const int ch = symm_chunk(m, tiles);
+            const i64 nch = (m + ch - 1) / ch;
+            wp = std::max(wp, nch * m * bb);
+        }
     }
+    const i64 wpHalf = (wp + 31) / 32 * 32;
+    if (b % ST == 0) wp = 2 * wpHalf;
     AdjWork w;
     i64 o = 0;
     auto take = [&](i64 cnt) {
@@ -529,6 +570,7 @@ AdjWork adj_layout(int n, int b) {
     w.parts = take(3 * (i64)n * n);
     w.grad = take(4);
     w.wp = take(std::max<i64>(wp, 1));
+    w.wp_half = wpHalf;
     w.wq = take((i64)std::max(1, n - 1) * bb);
     w.wg = take((i64)std::max(1, n - 1) * bb);
     w.t1 = take(bb);
@@ -578,58 +620,131 @@ int forward(hs_handle_t h, const double* X, i64 N, int d, int b, double ell, dou
     return HS_OK;
 }
 
-int reverse(hs_handle_t h, const double* L, double* Ab, i64 N, int b, double* ws, const AdjWork& w) {
-    const int n = (int)((N + b - 1) / b);
+// The per-step tail of reverse step k on the handle's stream, after the
+// trailing-adjoint partials of every panel block are in W (nch of them):
+// Pbar -= sum W, the TRSM and POTRF reversals.
+int reverse_tail(hs_handle_t h, const double* L, double* Ab, int b, int n, int k, int nch, const double* W,
+                 double* ws, const AdjWork& w) {
     const i64 bb = (i64)b * b;
     const int tilesN = b / AT, tiles = tilesN * tilesN;
-    const double* Dup = ws + w.dinv;             // D^{-T} per diagonal block
+    const int m = n - k - 1;
+    const double* Dup = ws + w.dinv;                // D^{-T} per diagonal block
     const double* Dlo = ws + w.dinv + (i64)n * bb;  // D^{-1}
+    double* Akk = Ab + hs_block_id(k, k, n) * bb;
+    const double* D = L + hs_block_id(k, k, n) * bb;
+    int rc;
+    if (m > 0) {
+        double* Pbar = Ab + hs_block_id(k + 1, k, n) * bb;     // panel adjoint, m blocks
+        const double* P = L + hs_block_id(k + 1, k, n) * bb;   // panel, m blocks
+        adj_sub_partials_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, W, m * bb, m * bb, nch);
+        HS_LAUNCH_CHECK();
+        // Abar_panel = Pbar D^{-1}
+        GemmArgs g1{Pbar, bb, b, 0, Dlo + (i64)k * bb, 0, b, ws + w.wq, bb, b, b, tilesN, tiles, 0};
+        if ((rc = launch_gemm(h, g1, m, tilesN))) return rc;
+        adj_copy_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wq, m * bb);
+        HS_LAUNCH_CHECK();
+        // Dbar -= tril(Abar_panel^T P): one partial per panel block in a
+        // single batched launch, summed in a fixed order
+        GemmArgs g2{Pbar, bb, b, 1, P, bb, b, ws + w.wg, bb, b, b, tilesN, tiles, 0};
+        if ((rc = launch_gemm(h, g2, m, tilesN))) return rc;
+        adj_gram_sub_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, m);
+        HS_LAUNCH_CHECK();
+    }
+    // POTRF reverse on block k
+    GemmArgs m1{D, 0, b, 1, Akk, 0, b, ws + w.t1, 0, b, b, tilesN, tiles, 1};   // Phi(D^T Dbar)
+    if ((rc = launch_gemm(h, m1, 1, tilesN))) return rc;
+    GemmArgs m2{Dup + (i64)k * bb, 0, b, 0, ws + w.t1, 0, b, ws + w.t2, 0, b, b, tilesN, tiles, 0};  // D^{-T} Phi
+    if ((rc = launch_gemm(h, m2, 1, tilesN))) return rc;
+    GemmArgs m3{ws + w.t2, 0, b, 0, Dlo + (i64)k * bb, 0, b, ws + w.t3, 0, b, b, tilesN, tiles, 0};  // .. D^{-1}
+    if ((rc = launch_gemm(h, m3, 1, tilesN))) return rc;
+    adj_symmetrize_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.t3, b);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
+int reverse(hs_handle_t h, const double* L, double* Ab, i64 N, int b, double* ws, const AdjWork& w) {
+    const int n = (int)((N + b - 1) / b);
     adj_init_kernel<<<dim3((unsigned)((i64)n * n)), dim3(256), 0, h->stream>>>(Ab, L, ws + w.v, ws + w.zs, N, b, n);
     HS_LAUNCH_CHECK();
     int rc;
-    for (int k = n - 1; k >= 0; --k) {
-        const int m = n - k - 1;
-        double* Akk = Ab + hs_block_id(k, k, n) * bb;
-        const double* D = L + hs_block_id(k, k, n) * bb;
-        if (m > 0) {
-            double* Pbar = Ab + hs_block_id(k + 1, k, n) * bb;     // panel adjoint, m blocks
-            const double* P = L + hs_block_id(k + 1, k, n) * bb;   // panel, m blocks
-            // Pbar -= (S + S^T) P
-            const bool big = (b % ST == 0);
-            const int stn = big ? b / ST : tilesN, stiles = stn * stn;
-            const int ch = symm_chunk(m, stiles);
-            const int nch = (m + ch - 1) / ch;
-            if (big)
-                adj_symm128_kernel<<<dim3((unsigned)((i64)nch * m * stiles)), dim3(256), 0, h->stream>>>(
-                    Ab, L, ws + w.wp, b, n, k, m, ch, stn);
-            else
-                adj_symm_kernel<<<dim3((unsigned)((i64)nch * m * stiles)), dim3(ATHREADS), 0, h->stream>>>(
-                    Ab, L, ws + w.wp, b, n, k, m, ch, stn);
-            HS_LAUNCH_CHECK();
-            adj_sub_partials_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wp, m * bb,
-                                                                                      m * bb, nch);
-            HS_LAUNCH_CHECK();
-            // Abar_panel = Pbar D^{-1}
-            GemmArgs g1{Pbar, bb, b, 0, Dlo + (i64)k * bb, 0, b, ws + w.wq, bb, b, b, tilesN, tiles, 0};
-            if ((rc = launch_gemm(h, g1, m, tilesN))) return rc;
-            adj_copy_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wq, m * bb);
-            HS_LAUNCH_CHECK();
-            // Dbar -= tril(Abar_panel^T P): one partial per panel block in a
-            // single batched launch, summed in a fixed order
-            GemmArgs g2{Pbar, bb, b, 1, P, bb, b, ws + w.wg, bb, b, b, tilesN, tiles, 0};
-            if ((rc = launch_gemm(h, g2, m, tilesN))) return rc;
-            adj_gram_sub_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, m);
-            HS_LAUNCH_CHECK();
+    if (b % ST != 0) {
+        // 64-tile product, one launch per step over every J, then the tail
+        const int tilesN = b / AT, tiles = tilesN * tilesN;
+        for (int k = n - 1; k >= 0; --k) {
+            const int m = n - k - 1;
+            int nch = 0;
+            if (m > 0) {
+                const int ch = symm_chunk(m, tiles);
+                nch = (m + ch - 1) / ch;
+                adj_symm_kernel<<<dim3((unsigned)((i64)nch * m * tiles)), dim3(ATHREADS), 0, h->stream>>>(
+                    Ab, L, ws + w.wp, b, n, k, m, ch, tilesN);
+                HS_LAUNCH_CHECK();
+            }
+            if ((rc = reverse_tail(h, L, Ab, b, n, k, nch, ws + w.wp, ws, w))) return rc;
         }
-        // POTRF reverse on block k
-        GemmArgs m1{D, 0, b, 1, Akk, 0, b, ws + w.t1, 0, b, b, tilesN, tiles, 1};   // Phi(D^T Dbar)
-        if ((rc = launch_gemm(h, m1, 1, tilesN))) return rc;
-        GemmArgs m2{Dup + (i64)k * bb, 0, b, 0, ws + w.t1, 0, b, ws + w.t2, 0, b, b, tilesN, tiles, 0};  // D^{-T} Phi
-        if ((rc = launch_gemm(h, m2, 1, tilesN))) return rc;
-        GemmArgs m3{ws + w.t2, 0, b, 0, Dlo + (i64)k * bb, 0, b, ws + w.t3, 0, b, b, tilesN, tiles, 0};  // .. D^{-1}
-        if ((rc = launch_gemm(h, m3, 1, tilesN))) return rc;
-        adj_symmetrize_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.t3, b);
+        return HS_OK;
+    }
+    // 128-tile path with a look-ahead of one step on the panel stream S2.
+    // Reverse step k needs the trailing adjoint S over the blocks (I, J),
+    // I, J > k; column k + 1 of it is final only when step k + 1 ends, the
+    // rest (I, J >= k + 2) already when step k + 2 ends. So the bulk of step
+    // k's product (rows and columns >= k + 2) runs on S2 beside step k + 1,
+    // and step k itself adds only block column k + 1 (all rows) and block
+    // row k + 1 (columns >= k + 2) before its tail. Events: C[j] = step j
+    // done (on S1), B[k] = bulk of step k done (on S2). The partial buffers
+    // alternate by step: bulk k - 2 starts after step k's tail has summed
+    // them. Step k's bulk reads only columns >= k + 2 while step k + 1
+    // writes column k + 1 and block (k + 1, k + 1).
+    const int stn = b / ST, stiles = stn * stn;
+    if ((rc = hs_ensure_events(h, 2 * (size_t)n + 1))) return rc;
+    hipEvent_t* C = h->events.data();
+    hipEvent_t* B = h->events.data() + n;
+    hipEvent_t start = h->events[2 * (size_t)n];
+    double* Wb[2] = {ws + w.wp, ws + w.wp + w.wp_half};
+    hipStream_t s1 = h->stream, s2 = h->stream2;
+    HS_HIP(hipEventRecord(start, s1));
+    HS_HIP(hipStreamWaitEvent(s2, start, 0));
+    // sp.nch work-groups per row of the first range, nch2 per row of the
+    // second (if any)
+    auto launch = [&](hipStream_t st, double* W, int k, const SymmPart& sp, int nch2) -> int {
+        const i64 units = (i64)sp.nrows * sp.nch + (sp.nrows2 > 0 ? (i64)sp.nrows2 * nch2 : 0);
+        if (units <= 0) return HS_OK;
+        const int m = n - k - 1;
+        adj_symm128_kernel<<<dim3((unsigned)(units * stiles)), dim3(256), 0, st>>>(Ab, L, W, b, n, k, m, sp, stn);
         HS_LAUNCH_CHECK();
+        return HS_OK;
+    };
+    for (int j = n - 1; j >= 0; --j) {
+        // bulk of step j - 1 on S2, beside step j
+        const int kb = j - 1, mb = n - kb - 1;
+        if (kb >= 0 && mb >= 2) {
+            if (j + 1 <= n - 1) HS_HIP(hipStreamWaitEvent(s2, C[j + 1], 0));
+            SymmPart sp{kb + 2, mb - 1, kb + 2, n, bulk_chunk(mb, stiles), 1};
+            sp.nch = bulk_chunks(mb, stiles);
+            if ((rc = launch(s2, Wb[kb & 1], kb, sp, 0))) return rc;
+            HS_HIP(hipEventRecord(B[kb], s2));
+        }
+        // step j on S1
+        const int m = n - j - 1;
+        const int nchB = bulk_chunks(m, stiles);
+        if (m >= 1) {
+            // block column j + 1 (every row, slot 0) and block row j + 1
+            // (columns j + 2 .., slots 1 .. nchB) in one launch
+            SymmPart thin{j + 1, m, j + 1, j + 2, 1, 0};
+            thin.nch = 1;
+            if (m >= 2) {
+                thin.I02 = j + 1;
+                thin.nrows2 = 1;
+                thin.J02 = j + 2;
+                thin.J12 = n;
+                thin.chunk2 = bulk_chunk(m, stiles);
+                thin.slot02 = 1;
+            }
+            if ((rc = launch(s1, Wb[j & 1], j, thin, nchB))) return rc;
+            if (m >= 2) HS_HIP(hipStreamWaitEvent(s1, B[j], 0));
+        }
+        if ((rc = reverse_tail(h, L, Ab, b, n, j, 1 + nchB, Wb[j & 1], ws, w))) return rc;
+        HS_HIP(hipEventRecord(C[j], s1));
     }
     return HS_OK;
 }
