@@ -785,7 +785,10 @@ extern "C" int hs_gp_nll_grad_f64(hs_handle_t h, const double* X, int64_t N, int
     double* ws = nullptr;
     if ((rc = hs_work(h, sizeof(double) * (size_t)w.total, (void**)&ws))) return rc;
     if ((rc = forward(h, X, N, d, b, ell, sf2, sn2, y, K, ws, w))) return rc;
-    if ((rc = reverse(h, K, Kbar, N, b, ws, w))) return rc;
+    if ((rc = reverse(h, K, Kbar, N, b, ws, w))) {
+        hipStreamSynchronize(h->stream2);  // no panel-stream work outlives a failed call
+        return rc;
+    }
     const size_t lds = sizeof(double) * 2 * (size_t)b * d;
     if (lds > 48 * 1024)
         HS_HIP(hipFuncSetAttribute((const void*)adj_contract_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

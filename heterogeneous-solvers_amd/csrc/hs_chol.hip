@@ -207,12 +207,9 @@ int hs_launch_trailing_on(hs_handle_t h, hipStream_t stream, double* A, int b, i
         if (mode >= 5) {
             trailing_mfma_f64<128, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         } else if (halfCu) {
-            static bool attr = false;
-            if (!attr) {
-                HS_HIP(hipFuncSetAttribute((const void*)trailing_mfma_f64<128, 0>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, HALF_CU_LDS));
-                attr = true;
-            }
+            // set per launch: the attribute belongs to the device current at the call
+            HS_HIP(hipFuncSetAttribute((const void*)trailing_mfma_f64<128, 0>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, HALF_CU_LDS));
             trailing_mfma_f64<128, 0><<<g, t, HALF_CU_LDS, stream>>>(A, b, n, pm);
         } else {
             trailing_mfma_f64<128, 0><<<g, t, 0, stream>>>(A, b, n, pm);
