@@ -4,7 +4,9 @@
  *
  * Plain C types only: device pointers are ordinary pointers obtained from
  * hs_malloc (or any hipMalloc'd memory), sizes are int64_t, every entry
- * returns an int status.
+ * returns an int status. Matrices must be ordinary (coarse-grained) device
+ * memory, not host-mapped or fine-grained allocations: the FP64 trailing
+ * update applies its result with device atomics performed in the L2.
  *
  * Storage contract (same as the reference's SymmetricMatrix,
  * src/dataStructures/SymmetricMatrix.hpp:9-35): the lower block triangle of
