@@ -201,7 +201,8 @@ def mixed_workload(hs, dev, syn, args, x64=None, rr64=None):
     if rr64:
         targets.append(("equal_residual", 1e-12, rr64))
     out = {}
-    for name, variant in (("fp32", hs.HS_MIXED_FP32), ("bf16x3", hs.HS_MIXED_BF16X3)):
+    for name, variant in (("fp32", hs.HS_MIXED_FP32), ("bf16x3", hs.HS_MIXED_BF16X3),
+                          ("f16x3", hs.HS_MIXED_F16X3)):
         runs = []
         for s in range(args.warmup + args.steps):
             t_fac.start()
@@ -494,6 +495,7 @@ def cpu_baseline(args):
 
 
 BF16_PEAK_TFLOPS = 2500.0
+FP16_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA, the same rate as BF16
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -523,6 +525,10 @@ def kernel_fractions(ch, mixed, cg, adj):
         out["trailing_bf16x3_ps256 + trailing_mfma_bf16x3 (BF16 hi+lo update)"] = {
             "achieved": b3, "achieved_three_products": 3 * b3, "unit": "TFLOP/s", "peak": BF16_PEAK_TFLOPS,
             "frac": b3 / BF16_PEAK_TFLOPS, "frac_three_products": 3 * b3 / BF16_PEAK_TFLOPS}
+        h3 = mixed["f16x3"]["update_tflops"]
+        out["trailing_f16x3_ps256 + trailing_mfma_f16x3 (scaled FP16 hi+lo update)"] = {
+            "achieved": h3, "achieved_three_products": 3 * h3, "unit": "TFLOP/s", "peak": FP16_PEAK_TFLOPS,
+            "frac": h3 / FP16_PEAK_TFLOPS, "frac_three_products": 3 * h3 / FP16_PEAK_TFLOPS}
     if cg:
         out["symv_partials (read-once SYMV block pass)"] = {
             "achieved": cg["symv_block_pass_gbs"], "unit": "GB/s", "peak": HBM_PEAK_GBS,

@@ -32,6 +32,9 @@ struct hs_handle_s {
     // block (up to 16 of them, b <= 256), written by the panel POTRF, read by
     // the panel TRSM that follows it on the same stream
     double* trsm_w = nullptr;
+    // the fp16 hi+lo split's scale of the mixed factor in progress: s, 1/s^2
+    // and the bound they come from (HS_MIXED_F16X3)
+    float* hilo_scale = nullptr;
 };
 
 struct hs_event_s {

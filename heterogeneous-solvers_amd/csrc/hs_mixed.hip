@@ -12,6 +12,17 @@
 //                       v_mfma_f32_16x16x32_bf16: hi*hi' + hi*lo' + lo*hi'
 //                       (the dropped lo*lo' and the split residuals are
 //                       ~2^-17 relative).
+//      HS_MIXED_F16X3:  the same three products on v_mfma_f32_16x16x32_f16
+//                       with an fp16 split of s*x, s a power of two that puts
+//                       the largest panel entry just under 2^15: fp16 keeps
+//                       11 significand bits to bf16's 8, so hi + lo holds
+//                       ~22 bits and the update is near FP32 accuracy at the
+//                       BF16 MFMA rate (the FP16 hi+lo construction of Ootomo
+//                       and Yokota, "Recovering single precision accuracy from
+//                       Tensor Cores while surpassing the FP32 theoretical
+//                       peak performance", IJHPCA 2022, with one power-of-two
+//                       scale of the whole factor in place of their scaled lo
+//                       term). The accumulator is scaled back by 1/s^2, exact.
 // The steps follow the same one-panel look-ahead schedule as the FP64 path.
 #include <algorithm>
 #include <cstdlib>
@@ -187,32 +198,82 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_f32(float* __restrict__
     epilogue_f32<MT, MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
-// ---- BF16 hi+lo split trailing update -------------------------------------
+// ---- hi+lo split trailing updates (BF16 and scaled FP16) -------------------
 __device__ inline unsigned short bf16_rn(float x) {
     const unsigned u = __float_as_uint(x);
     return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
 
+// The 16-bit format of a split: rn rounds an FP32 value to it (nearest,
+// ties to even), to_f widens it back exactly, mfma is the 16x16x32 MFMA
+// with FP32 accumulation. An fp16 split works on s*x (Hilo::scaled): sc[0]
+// holds s and sc[1] 1/s^2, both powers of two (hilo_scale_finish_kernel).
+enum class H16 { BF16, F16 };
+template <H16 F>
+struct Hilo;
+template <>
+struct Hilo<H16::BF16> {
+    static constexpr bool scaled = false;
+    typedef __bf16 frag __attribute__((ext_vector_type(8)));
+    static __device__ inline unsigned short rn(float x) { return bf16_rn(x); }
+    static __device__ inline float to_f(unsigned short h) { return bf16_to_f(h); }
+    static __device__ inline f4 mfma(frag a, frag b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Hilo<H16::F16> {
+    static constexpr bool scaled = true;
+    typedef _Float16 frag __attribute__((ext_vector_type(8)));
+    static __device__ inline unsigned short rn(float x) { return __builtin_bit_cast(unsigned short, (_Float16)x); }
+    static __device__ inline float to_f(unsigned short h) { return (float)__builtin_bit_cast(_Float16, h); }
+    static __device__ inline f4 mfma(frag a, frag b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// hi = rn(s x), lo = rn(s x - hi): s x and the difference are exact in FP32
+template <H16 F>
+__device__ inline void hilo_split(float x, float s, unsigned short* hi, unsigned short* lo) {
+    const float xs = Hilo<F>::scaled ? x * s : x;
+    const unsigned short h = Hilo<F>::rn(xs);
+    *hi = h;
+    *lo = Hilo<F>::rn(xs - Hilo<F>::to_f(h));
+}
+
+// the accumulated products of scaled operands back to the matrix's scale
+template <H16 F, int MI, int MJ>
+__device__ inline void hilo_unscale(f4 (&acc)[MI][MJ], const float* sc) {
+    if constexpr (Hilo<F>::scaled) {
+        const float r = sc[1];
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < MJ; ++j) acc[i][j] *= r;
+    }
+}
+
 // TILE x TILE tile per work-group, 2x2 waves of (TILE/2)^2, KT = 32 (one
-// bf16 MFMA depth) per LDS stage, double buffered. LDS keeps hi and lo
+// 16-bit MFMA depth) per LDS stage, double buffered. LDS keeps hi and lo
 // planes row-major [row][k] with an 80-byte row pitch, so the 16-byte
 // fragment reads of 16 lanes (8 consecutive k of 16 rows) hit distinct banks.
 // Fragment of lane l: row l%16, k = 8(l/16)..+7; output as the f32 MFMA.
 //
 // PS (pre-split): the operand blocks come from hi / lo planes that
-// split_panels_bf16_kernel wrote once for the whole launch (Wh, Wl: np
-// panels x m block rows from rs, b x b bf16 each), so the stage fill is a
+// split_panels_kernel wrote once for the whole launch (Wh, Wl: np panels x
+// m block rows from rs, b x b 16-bit values each), so the stage fill is a
 // copy; otherwise every work-group splits the FP32 values it loads, which
 // costs more VALU issue than the MFMAs of the stage.
-template <int TILE, int KIND, bool PS>
-__global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restrict__ A, int b, int n, PairMap pm,
-                                                                 const unsigned short* __restrict__ Wh,
-                                                                 const unsigned short* __restrict__ Wl, int rs,
-                                                                 int m) {
+template <int TILE, bool PS, H16 F>
+__device__ __forceinline__ void trailing_x3_body(float* __restrict__ A, int b, int n, const PairMap& pm,
+                                                 const unsigned short* __restrict__ Wh,
+                                                 const unsigned short* __restrict__ Wl, int rs, int m,
+                                                 const float* __restrict__ sc) {
+    typedef typename Hilo<F>::frag bf8;
     constexpr int KT = 32;
     constexpr int WT = TILE / 2, MT = WT / 16;
-    constexpr int P = 40;                 // row pitch in bf16 elements (80 bytes)
+    constexpr int P = 40;                 // row pitch in 16-bit elements (80 bytes)
     constexpr int PLANE = TILE * P;       // one hi or lo plane of one operand
     constexpr int STAGE = 4 * PLANE;      // A hi, A lo, B hi, B lo
     __shared__ unsigned short lds[2 * STAGE];
@@ -268,17 +329,14 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
             }
         }
     };
+    const float scale = (!PS && Hilo<F>::scaled) ? sc[0] : 1.0f;
     auto split_store = [&](const float4* r, unsigned short* hiPlane, unsigned short* loPlane) {
         unsigned short hi[LPT], lo[LPT];
 #pragma unroll
         for (int v = 0; v < LPT / 4; ++v) {
             const float x[4] = {r[v].x, r[v].y, r[v].z, r[v].w};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const unsigned short h = bf16_rn(x[c]);
-                hi[4 * v + c] = h;
-                lo[4 * v + c] = bf16_rn(x[c] - bf16_to_f(h));
-            }
+            for (int c = 0; c < 4; ++c) hilo_split<F>(x[c], scale, &hi[4 * v + c], &lo[4 * v + c]);
         }
         uint4* dh = reinterpret_cast<uint4*>(hiPlane + lrow * P + lk);
         uint4* dl = reinterpret_cast<uint4*>(loPlane + lrow * P + lk);
@@ -343,13 +401,31 @@ __global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restric
         for (int i = 0; i < MT; ++i)
 #pragma unroll
             for (int j = 0; j < MT; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = Hilo<F>::mfma(al[i], bh[j], acc[i][j]);
+                acc[i][j] = Hilo<F>::mfma(ah[i], bl[j], acc[i][j]);
+                acc[i][j] = Hilo<F>::mfma(ah[i], bh[j], acc[i][j]);
             }
         __syncthreads();
     }
+    hilo_unscale<F>(acc, sc);
     epilogue_f32<MT, MT>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+}
+
+// KIND only separates the look-ahead column launches (KIND 1, modes 5 and 6)
+// from the rest in profiles; the code is the same.
+template <int TILE, int KIND, bool PS>
+__global__ __launch_bounds__(THREADS) void trailing_mfma_bf16x3(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                 const unsigned short* __restrict__ Wh,
+                                                                 const unsigned short* __restrict__ Wl, int rs,
+                                                                 int m) {
+    trailing_x3_body<TILE, PS, H16::BF16>(A, b, n, pm, Wh, Wl, rs, m, nullptr);
+}
+template <int TILE, int KIND, bool PS>
+__global__ __launch_bounds__(THREADS) void trailing_mfma_f16x3(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                const unsigned short* __restrict__ Wh,
+                                                                const unsigned short* __restrict__ Wl, int rs, int m,
+                                                                const float* __restrict__ sc) {
+    trailing_x3_body<TILE, PS, H16::F16>(A, b, n, pm, Wh, Wl, rs, m, sc);
 }
 
 // Rest update from pre-split planes in 256 x 256 tiles (b % 256 == 0): 512
@@ -377,10 +453,12 @@ constexpr size_t T2LDS = 2 * (size_t)T2STAGE;
 constexpr unsigned WAIT_VM0_LGKM0 = 0x0070;         // s_waitcnt vmcnt(0) lgkmcnt(0)
 constexpr unsigned WAIT_VM0 = 0x0F70;               // s_waitcnt vmcnt(0)
 
-__global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __restrict__ A, int b, int n, PairMap pm,
-                                                                   const unsigned short* __restrict__ Wh,
-                                                                   const unsigned short* __restrict__ Wl, int rs,
-                                                                   int m) {
+template <H16 F>
+__device__ __forceinline__ void trailing_x3_ps256_body(float* __restrict__ A, int b, int n, const PairMap& pm,
+                                                       const unsigned short* __restrict__ Wh,
+                                                       const unsigned short* __restrict__ Wl, int rs, int m,
+                                                       const float* __restrict__ sc) {
+    typedef typename Hilo<F>::frag bf8;
     constexpr int KT = 32;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds2[];
     const int tpe = b / T2, tpb = tpe * tpe;
@@ -437,9 +515,9 @@ __global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __rest
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 f4& c = acc[i][4 * h + j];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
+                c = Hilo<F>::mfma(al[i], bh[j], c);
+                c = Hilo<F>::mfma(ah[i], bl[j], c);
+                c = Hilo<F>::mfma(ah[i], bh[j], c);
             }
     };
     dma(0, 0);
@@ -483,7 +561,21 @@ __global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __rest
         __builtin_amdgcn_sched_group_barrier(0x008, 16, 1);
     }
     __builtin_amdgcn_s_waitcnt(WAIT_VM0);  // no DMA into LDS outlives the work-group
+    hilo_unscale<F>(acc, sc);
     epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
+}
+
+__global__ __launch_bounds__(T2THREADS) void trailing_bf16x3_ps256(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                   const unsigned short* __restrict__ Wh,
+                                                                   const unsigned short* __restrict__ Wl, int rs,
+                                                                   int m) {
+    trailing_x3_ps256_body<H16::BF16>(A, b, n, pm, Wh, Wl, rs, m, nullptr);
+}
+__global__ __launch_bounds__(T2THREADS) void trailing_f16x3_ps256(float* __restrict__ A, int b, int n, PairMap pm,
+                                                                  const unsigned short* __restrict__ Wh,
+                                                                  const unsigned short* __restrict__ Wl, int rs,
+                                                                  int m, const float* __restrict__ sc) {
+    trailing_x3_ps256_body<H16::F16>(A, b, n, pm, Wh, Wl, rs, m, sc);
 }
 
 // FP32 rest update in 256 x 256 tiles (b % 256 == 0), the FP32 sibling of
@@ -587,12 +679,14 @@ __global__ __launch_bounds__(T2THREADS) void trailing_f32_ps256(float* __restric
     epilogue_f32<4, 8>(C, b, wm0, wn0, lr, lq, diag && tm == tn, acc);
 }
 
-// The bf16 hi / lo planes of np panels (block columns k .. k+np-1) for the
-// m block rows from rs: W[(p m + i) b^2 + e] for block (rs + i, k + p),
-// hi = rn(x), lo = rn(x - hi), eight values per thread.
-__global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __restrict__ L, int b, int n, int k,
-                                                                int np, int rs, int m, unsigned short* __restrict__ Wh,
-                                                                unsigned short* __restrict__ Wl) {
+// The hi / lo planes of np panels (block columns k .. k+np-1) for the m
+// block rows from rs: W[(p m + i) b^2 + e] for block (rs + i, k + p),
+// hilo_split of each value, eight values per thread.
+template <H16 F>
+__device__ __forceinline__ void split_panels_body(const float* __restrict__ L, int b, int n, int k, int np, int rs,
+                                                  int m, unsigned short* __restrict__ Wh,
+                                                  unsigned short* __restrict__ Wl, const float* __restrict__ sc) {
+    const float scale = Hilo<F>::scaled ? sc[0] : 1.0f;
     const i64 bb = (i64)b * b;
     const i64 total = (i64)np * m * bb;
     for (i64 e = ((i64)blockIdx.x * blockDim.x + threadIdx.x) * 8; e < total; e += (i64)gridDim.x * blockDim.x * 8) {
@@ -605,9 +699,9 @@ __global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __r
         unsigned hi[4], lo[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const unsigned short h0 = bf16_rn(x[2 * c]), h1 = bf16_rn(x[2 * c + 1]);
-            const unsigned short l0 = bf16_rn(x[2 * c] - bf16_to_f(h0));
-            const unsigned short l1 = bf16_rn(x[2 * c + 1] - bf16_to_f(h1));
+            unsigned short h0, h1, l0, l1;
+            hilo_split<F>(x[2 * c], scale, &h0, &l0);
+            hilo_split<F>(x[2 * c + 1], scale, &h1, &l1);
             hi[c] = h0 | ((unsigned)h1 << 16);
             lo[c] = l0 | ((unsigned)l1 << 16);
         }
@@ -616,11 +710,86 @@ __global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __r
     }
 }
 
+__global__ __launch_bounds__(256) void split_panels_bf16_kernel(const float* __restrict__ L, int b, int n, int k,
+                                                                int np, int rs, int m, unsigned short* __restrict__ Wh,
+                                                                unsigned short* __restrict__ Wl) {
+    split_panels_body<H16::BF16>(L, b, n, k, np, rs, m, Wh, Wl, nullptr);
+}
+__global__ __launch_bounds__(256) void split_panels_f16_kernel(const float* __restrict__ L, int b, int n, int k,
+                                                               int np, int rs, int m, unsigned short* __restrict__ Wh,
+                                                               unsigned short* __restrict__ Wl,
+                                                               const float* __restrict__ sc) {
+    split_panels_body<H16::F16>(L, b, n, k, np, rs, m, Wh, Wl, sc);
+}
+
+// The fp16 split's scale. Every entry of a Cholesky factor of A satisfies
+// |L_ij| <= sqrt(A_ii), so one scale serves all of a factorisation's panels:
+// hilo_diag_max_kernel takes the largest diagonal entry of the FP32-stored
+// matrix (rows below N; single work-group), hilo_abs_max_kernel the largest
+// |entry| of one panel (block column k, rows k+1 .. n-1) for the per-step
+// entry; both leave a non-negative bound in sc[2] (as its bits: for
+// non-negative floats the unsigned order is the numeric order, so work-groups
+// combine by atomicMax). hilo_scale_finish_kernel then writes s = 2^(15 - E)
+// with bound = f 2^E, f in [0.5, 1), so that s * bound < 2^15 (the fp16
+// range ends at 65504), and sc[1] = 1 / s^2. The exponent shift is clamped to
+// [-60, 60] so that 1 / s^2 stays a normal FP32 number.
+__global__ __launch_bounds__(256) void hilo_diag_max_kernel(const float* __restrict__ L, int b, int n, i64 N,
+                                                            float* __restrict__ sc) {
+    __shared__ float red[256];
+    float mx = 0.0f;
+    for (i64 i = threadIdx.x; i < N; i += 256) {
+        const i64 J = i / b, r = i % b;
+        mx = fmaxf(mx, L[hs_block_id(J, J, n) * (i64)b * b + r * b + r]);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sc[2] = sqrtf(red[0]);
+}
+
+__global__ void hilo_zero_kernel(float* __restrict__ sc) {
+    if (threadIdx.x < 4) sc[threadIdx.x] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void hilo_abs_max_kernel(const float* __restrict__ A, int b, int n, int k,
+                                                           float* __restrict__ sc) {
+    __shared__ float red[256];
+    const i64 bb = (i64)b * b;
+    const i64 first = hs_block_id(k + 1, k, n) * bb, total = (i64)(n - k - 1) * bb;
+    float mx = 0.0f;
+    for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < total; e += (i64)gridDim.x * 256)
+        mx = fmaxf(mx, fabsf(A[first + e]));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned*>(sc + 2), __float_as_uint(red[0]));
+}
+
+__global__ void hilo_scale_finish_kernel(float* __restrict__ sc) {
+    if (threadIdx.x != 0) return;
+    const float bound = sc[2];
+    int e = 0;
+    if (bound > 0.0f && bound <= 3.0e38f) frexpf(bound, &e);
+    const int sh = min(60, max(-60, 15 - e));
+    sc[0] = ldexpf(1.0f, sh);
+    sc[1] = ldexpf(1.0f, -2 * sh);
+}
+
 }  // namespace
 
-// planes (BF16 hi+lo only): the np panels k .. k+np-1 as bf16 hi / lo planes
-// for the block rows from r0 (split_panels_bf16_kernel layout, m = n - r0);
-// null for an update that splits the FP32 values it loads.
+// The dynamic LDS limit of a 256-tile kernel, set on the device current at
+// the launch (the attribute belongs to that device).
+static int set_lds(const void* fn, size_t bytes) {
+    HS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return HS_OK;
+}
+
 // FP32 updates of several panels, and whole-triangle steps, in 256 x 256
 // tiles (HS_F32_TILE256=0 keeps the 128 x 128 kernel)
 static const bool g_f32_ps256 = [] {
@@ -628,8 +797,13 @@ static const bool g_f32_ps256 = [] {
     return !(e && e[0] == '0');
 }();
 
+// planes (hi+lo variants only): the np panels k .. k+np-1 as hi / lo planes
+// for the block rows from r0 (split_panels_kernel layout, m = n - r0); null
+// for an update that splits the FP32 values it loads. sc: the fp16 split's
+// scale (HS_MIXED_F16X3 only).
 int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b, int k, int rowStart, int rowCount,
-                             int n, int mode, int variant, int np, const unsigned short* planes, int r0) {
+                             int n, int mode, int variant, int np, const unsigned short* planes, int r0,
+                             const float* sc) {
     if (rowCount <= 0) return HS_OK;
     if (b % 128 != 0) {
         hs_set_error("mixed-precision trailing update needs b % 128 == 0");
@@ -645,44 +819,47 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
         return HS_ERR_UNSUPPORTED;
     }
     const dim3 g((unsigned)grid), t(THREADS);
-    if (variant == HS_MIXED_BF16X3) {
+    const bool f16 = variant == HS_MIXED_F16X3;
+    if (variant == HS_MIXED_BF16X3 || f16) {
         if (planes) {
             const int m = n - r0;
             const unsigned short* Wh = planes;
             const unsigned short* Wl = planes + (i64)np * m * b * b;
             if (b % T2 == 0) {
-                static bool attr = false;
-                if (!attr) {
-                    HS_HIP(hipFuncSetAttribute((const void*)trailing_bf16x3_ps256,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)T2LDS));
-                    attr = true;
-                }
                 PairMap pm2;
                 const i64 grid2 = hs_pair_map(k, rowStart, rowCount, n, mode, np, (i64)(b / T2) * (b / T2), &pm2);
-                if (grid2 > 0)
-                    trailing_bf16x3_ps256<<<dim3((unsigned)grid2), dim3(T2THREADS), T2LDS, stream>>>(A, b, n, pm2, Wh,
-                                                                                                   Wl, r0, m);
-            } else if (mode >= 5) {
-                trailing_mfma_bf16x3<TILE, 1, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
+                const dim3 g2((unsigned)grid2), t2(T2THREADS);
+                if (grid2 > 0 && f16) {
+                    if (int rc = set_lds((const void*)trailing_f16x3_ps256, T2LDS)) return rc;
+                    trailing_f16x3_ps256<<<g2, t2, T2LDS, stream>>>(A, b, n, pm2, Wh, Wl, r0, m, sc);
+                } else if (grid2 > 0) {
+                    if (int rc = set_lds((const void*)trailing_bf16x3_ps256, T2LDS)) return rc;
+                    trailing_bf16x3_ps256<<<g2, t2, T2LDS, stream>>>(A, b, n, pm2, Wh, Wl, r0, m);
+                }
+            } else if (f16) {
+                if (mode >= 5) trailing_mfma_f16x3<TILE, 1, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m, sc);
+                else trailing_mfma_f16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m, sc);
             } else {
-                trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
+                if (mode >= 5) trailing_mfma_bf16x3<TILE, 1, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
+                else trailing_mfma_bf16x3<TILE, 0, true><<<g, t, 0, stream>>>(A, b, n, pm, Wh, Wl, r0, m);
             }
+        } else if (f16) {
+            if (mode >= 5)
+                trailing_mfma_f16x3<TILE, 1, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0, sc);
+            else
+                trailing_mfma_f16x3<TILE, 0, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0, sc);
         } else if (mode >= 5) {
             trailing_mfma_bf16x3<TILE, 1, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
         } else {
             trailing_mfma_bf16x3<TILE, 0, false><<<g, t, 0, stream>>>(A, b, n, pm, nullptr, nullptr, 0, 0);
         }
     } else if (b % T2 == 0 && (np > 1 || mode == 3) && g_f32_ps256) {
-        static bool attr = false;
-        if (!attr) {
-            HS_HIP(hipFuncSetAttribute((const void*)trailing_f32_ps256, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)F2LDS));
-            attr = true;
-        }
         PairMap pm2;
         const i64 grid2 = hs_pair_map(k, rowStart, rowCount, n, mode, np, (i64)(b / T2) * (b / T2), &pm2);
-        if (grid2 > 0)
+        if (grid2 > 0) {
+            if (int rc = set_lds((const void*)trailing_f32_ps256, F2LDS)) return rc;
             trailing_f32_ps256<<<dim3((unsigned)grid2), dim3(T2THREADS), F2LDS, stream>>>(A, b, n, pm2);
+        }
     } else {
         if (mode >= 5) trailing_mfma_f32<TILE, 1><<<g, t, 0, stream>>>(A, b, n, pm);
         else trailing_mfma_f32<TILE, 0><<<g, t, 0, stream>>>(A, b, n, pm);
@@ -691,15 +868,36 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
     return HS_OK;
 }
 
+// The fp16 split's scale for a factorisation of the FP32-stored matrix L
+// (from its largest diagonal entry), into h->hilo_scale on h->stream.
+static int hilo_scale_for_factor(hs_handle_t h, const float* L, int b, int n, i64 N) {
+    hilo_diag_max_kernel<<<1, 256, 0, h->stream>>>(L, b, n, N, h->hilo_scale);
+    HS_LAUNCH_CHECK();
+    hilo_scale_finish_kernel<<<1, 64, 0, h->stream>>>(h->hilo_scale);
+    HS_LAUNCH_CHECK();
+    return HS_OK;
+}
+
 // The fused trailing update of one right-looking step on FP32 storage, in
-// either variant (the mixed factor's per-op entry points).
+// any variant (the mixed factor's per-op entry points). The fp16 split takes
+// its scale from the step's own panel.
 static int trailing_mixed_step(hs_handle_t h, float* A, int b, int k, int blockCountXY, int variant) {
     if (!h) return -1;
     if (!A) return -2;
     if (b <= 0) return -3;
     if (k < 0 || k >= blockCountXY) return -4;
+    if (variant == HS_MIXED_F16X3 && b % 128 == 0 && k + 1 < blockCountXY) {
+        const i64 total = (i64)(blockCountXY - k - 1) * b * b;
+        const unsigned G = (unsigned)std::min<i64>(1024, (total + 4095) / 4096);
+        hilo_zero_kernel<<<1, 64, 0, h->stream>>>(h->hilo_scale);
+        HS_LAUNCH_CHECK();
+        hilo_abs_max_kernel<<<G, 256, 0, h->stream>>>(A, b, blockCountXY, k, h->hilo_scale);
+        HS_LAUNCH_CHECK();
+        hilo_scale_finish_kernel<<<1, 64, 0, h->stream>>>(h->hilo_scale);
+        HS_LAUNCH_CHECK();
+    }
     return hs_launch_trailing_mixed(h, h->stream, A, b, k, k + 1, blockCountXY - k - 1, blockCountXY, 3, variant, 1,
-                                    nullptr, 0);
+                                    nullptr, 0, h->hilo_scale);
 }
 
 extern "C" int hs_syrk_gemm_trailing_f32(hs_handle_t h, float* A, int b, int k, int blockCountXY) {
@@ -708,6 +906,10 @@ extern "C" int hs_syrk_gemm_trailing_f32(hs_handle_t h, float* A, int b, int k, 
 
 extern "C" int hs_syrk_gemm_trailing_bf16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY) {
     return trailing_mixed_step(h, A, b, k, blockCountXY, HS_MIXED_BF16X3);
+}
+
+extern "C" int hs_syrk_gemm_trailing_f16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY) {
+    return trailing_mixed_step(h, A, b, k, blockCountXY, HS_MIXED_F16X3);
 }
 
 extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, double shift) {
@@ -728,13 +930,16 @@ extern "C" int hs_to_f32_packed(hs_handle_t h, const double* K, float* L32, int6
     return HS_OK;
 }
 
-// bf16 hi / lo planes of the np panels k .. k+np-1 for the m block rows from
-// r0 (W: hi planes, then lo planes)
+// hi / lo planes of the np panels k .. k+np-1 for the m block rows from r0
+// (W: hi planes, then lo planes), bf16 or scaled fp16 per variant
 static int split_panels(hipStream_t stream, const float* L, int b, int n, int k, int np, int r0, int m,
-                        unsigned short* W) {
+                        unsigned short* W, int variant, const float* sc) {
     const i64 half = (i64)np * m * b * b;
     const i64 sg = std::min<i64>(8192, (half / 8 + 255) / 256);
-    split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(L, b, n, k, np, r0, m, W, W + half);
+    if (variant == HS_MIXED_F16X3)
+        split_panels_f16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(L, b, n, k, np, r0, m, W, W + half, sc);
+    else
+        split_panels_bf16_kernel<<<dim3((unsigned)sg), dim3(256), 0, stream>>>(L, b, n, k, np, r0, m, W, W + half);
     HS_LAUNCH_CHECK();
     return HS_OK;
 }
@@ -757,7 +962,7 @@ static int mixed_panels(int n, int variant) {
     return std::max(1, std::min(v, n / 2));
 }
 
-// the BF16 hi+lo factor's two plane sets (the FP32 one needs no workspace)
+// the hi+lo factors' two plane sets (the FP32 one needs no workspace)
 size_t hs_ws_mixed(i64 N, int b) {
     const int n = (int)((N + b - 1) / b);
     const i64 planeSet = 2 * (i64)mixed_panels(n, HS_MIXED_BF16X3) * n * b * b;
@@ -769,7 +974,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     int rc = hs_reset_info(h);
     if (rc) return rc;
     const int P = mixed_panels(n, variant);
-    // BF16 hi+lo: the P panels of a step as bf16 hi / lo planes for the block
+    // hi+lo variants: the P panels of a step as hi / lo planes for the block
     // rows below them (4 bytes per element), split once on the panel stream
     // when the panels are final and read by both the step's column update
     // (S2) and its rest update (S1). Two sets alternate by step: the set of
@@ -777,13 +982,17 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
     // waited for S1's update of step s.
     unsigned short* planes = nullptr;
     const i64 planeSet = 2 * (i64)P * n * b * b;
-    if (variant == HS_MIXED_BF16X3) {
+    if (variant == HS_MIXED_BF16X3 || variant == HS_MIXED_F16X3) {
         if ((rc = hs_work(h, sizeof(unsigned short) * 2 * (size_t)planeSet, (void**)&planes))) return rc;
     }
+    // the fp16 split's scale, on the ordering stream before the schedule
+    // starts (the panel stream waits for everything queued there)
+    if (variant == HS_MIXED_F16X3 && (rc = hilo_scale_for_factor(h, L32, b, n, N))) return rc;
+    const float* sc = h->hilo_scale;
     auto planesOf = [&](int k) { return planes + ((k / P) & 1) * planeSet; };
     auto prep = [&](hipStream_t s, int k0) -> int {
         if (!planes || !hs_plane_prep_needed(k0, P, n)) return HS_OK;
-        return split_panels(s, L32, b, n, k0, P, k0 + P, n - (k0 + P), planesOf(k0));
+        return split_panels(s, L32, b, n, k0, P, k0 + P, n - (k0 + P), planesOf(k0), variant, sc);
     };
     auto potrf = [&](int k) { return hs_factor_potrf<float>(h, L32, N, b, hs_block_id(k, k, n), k); };
     auto trsm = [&](int k, int start, int count) {
@@ -793,7 +1002,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
         // the step's rest update and its P-panel column update read the split planes
         const bool shared = planes && hs_trail_reads_planes(mode, np, P);
         return hs_launch_trailing_mixed(h, s, L32, b, k, rowStart, rowCount, n, mode, variant, np,
-                                        shared ? planesOf(k) : nullptr, k + P);
+                                        shared ? planesOf(k) : nullptr, k + P, sc);
     };
     if ((rc = hs_lookahead_factor(h, n, pt, potrf, trsm, trail, P, prep))) {
         hipStreamSynchronize(h->stream2);
@@ -809,7 +1018,7 @@ static int mixed_factor(hs_handle_t h, float* L32, i64 N, int b, int variant, Ph
 extern "C" int hs_cholesky_mixed_schedule(int64_t N, int b, int variant, int64_t* rest_launches, double* rest_flops) {
     if (N <= 0) return -1;
     if (b % 128 != 0 || b > 256) return -2;
-    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -3;
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3 && variant != HS_MIXED_F16X3) return -3;
     if (!rest_launches) return -4;
     if (!rest_flops) return -5;
     const int n = (int)((N + b - 1) / b);
@@ -827,7 +1036,7 @@ extern "C" int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int
         hs_set_error("hs_cholesky_mixed: matrix_bsz must be 128 or 256");
         return -5;
     }
-    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -6;
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3 && variant != HS_MIXED_F16X3) return -6;
     int rc = hs_to_f32_packed(h, K, L32, N, b, shift);
     if (rc) return rc;
     PhaseTimer pt;
@@ -846,7 +1055,7 @@ extern "C" int hs_cholesky_mixed_timed(hs_handle_t h, const double* K, float* L3
         hs_set_error("hs_cholesky_mixed: matrix_bsz must be 128 or 256");
         return -5;
     }
-    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3) return -6;
+    if (variant != HS_MIXED_FP32 && variant != HS_MIXED_BF16X3 && variant != HS_MIXED_F16X3) return -6;
     int rc = hs_to_f32_packed(h, K, L32, N, b, shift);
     if (rc) return rc;
     PhaseTimer pt;

@@ -48,6 +48,7 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
     if (e == hipSuccess) e = hipMemset(h->info_dev, 0, sizeof(int) * 2);
     if (e == hipSuccess) e = hipHostMalloc(&h->pinned, 4096, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&h->trsm_w, sizeof(double) * 16 * 256);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->hilo_scale, sizeof(float) * 64);
     for (int i = 0; i < HS_HANDLE_EVENTS && e == hipSuccess; ++i) {
         hipEvent_t ev;
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
@@ -57,6 +58,10 @@ extern "C" int hs_handle_create(hs_handle_t* handle, int device) {
     if (e == hipSuccess && hs_noalloc_setup(device) != HS_OK) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         for (auto ev : h->events) hipEventDestroy(ev);
+        if (h->hilo_scale) hipFree(h->hilo_scale);
+        if (h->trsm_w) hipFree(h->trsm_w);
+        if (h->pinned) hipHostFree(h->pinned);
+        if (h->info_dev) hipFree(h->info_dev);
         hipStreamDestroy(h->stream2);
         hipStreamDestroy(h->stream);
         delete h;
@@ -76,6 +81,7 @@ extern "C" int hs_handle_destroy(hs_handle_t h) {
     if (h->work) hipFree(h->work);
     if (h->info_dev) hipFree(h->info_dev);
     if (h->trsm_w) hipFree(h->trsm_w);
+    if (h->hilo_scale) hipFree(h->hilo_scale);
     if (h->pinned) hipHostFree(h->pinned);
     hipStreamDestroy(h->stream);
     delete h;

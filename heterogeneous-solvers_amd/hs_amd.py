@@ -97,6 +97,7 @@ _SIGS = {
     "hs_cholesky_mixed": [_P, _P, _P, _L, _I, _I, _D],
     "hs_syrk_gemm_trailing_f32": [_P, _P, _I, _I, _I],
     "hs_syrk_gemm_trailing_bf16x3": [_P, _P, _I, _I, _I],
+    "hs_syrk_gemm_trailing_f16x3": [_P, _P, _I, _I, _I],
     "hs_cholesky_mixed_timed": [_P, _P, _P, _L, _I, _I, _D, C.POINTER(_D), C.POINTER(_L)],
     "hs_refine_solve": [_P, _P, _P, _P, _P, _L, _I, _D, _I, _I, _P],
     "hs_refine_solve_to": [_P, _P, _P, _P, _P, _L, _I, _D, _D, _I, _I, _P],
@@ -111,6 +112,7 @@ _SIGS = {
 
 HS_MIXED_FP32 = 0
 HS_MIXED_BF16X3 = 1
+HS_MIXED_F16X3 = 2
 
 
 class RefineReport(C.Structure):

@@ -344,6 +344,9 @@ int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* 
 
 #define HS_MIXED_FP32 0   /* trailing updates on FP32 MFMA */
 #define HS_MIXED_BF16X3 1 /* trailing updates on BF16 MFMA, hi+lo split, 3 products */
+#define HS_MIXED_F16X3 2  /* trailing updates on FP16 MFMA, hi+lo split of the panels
+                             scaled by a power of two (largest entry under 2^15),
+                             3 products: ~22-bit operands, near FP32 accuracy */
 
 /* Low-precision factor of the FP64 packed K: L32 (FP32 packed, same layout)
  * <- chol(K + shift I). Diagonal POTRF and panel TRSM in FP64 arithmetic on
@@ -352,13 +355,15 @@ int hs_rbf_window_f64(hs_handle_t h, const double* t, int64_t N, int b, double* 
 int hs_cholesky_mixed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift);
 /* The fused trailing update of right-looking step k on an FP32-stored packed
  * factor (SYRK on every diagonal block, GEMM on every off-diagonal block of
- * the trailing triangle, k < J <= I < n): FP32 MFMA, or BF16 hi+lo with three
- * products, accumulated in FP32; b % 128 == 0. The mixed-precision siblings of
- * hs_syrk_gemm_trailing_f64 (symmetricMatrixMatrixDiagonal_optimizedGPU +
+ * the trailing triangle, k < J <= I < n): FP32 MFMA, or BF16 / scaled FP16
+ * hi+lo with three products, accumulated in FP32; b % 128 == 0 (the FP16
+ * entry scales by the largest |entry| of panel k). The mixed-precision
+ * siblings of hs_syrk_gemm_trailing_f64 (symmetricMatrixMatrixDiagonal_optimizedGPU +
  * matrixMatrixStep_optimizedGPU3 of the reference's FP32 build,
  * MatrixMatrixOperations.cpp:212-304, 645-778). */
 int hs_syrk_gemm_trailing_f32(hs_handle_t h, float* A, int b, int k, int blockCountXY);
 int hs_syrk_gemm_trailing_bf16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY);
+int hs_syrk_gemm_trailing_f16x3(hs_handle_t h, float* A, int b, int k, int blockCountXY);
 /* Same with per-phase kernel times (as hs_cholesky_f64_timed). */
 int hs_cholesky_mixed_timed(hs_handle_t h, const double* K, float* L32, int64_t N, int b, int variant, double shift,
                             double* phase_ms, int64_t* launches);

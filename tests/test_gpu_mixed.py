@@ -1,5 +1,5 @@
-"""Mixed-precision factor (FP32 storage; FP32-MFMA or BF16 hi+lo trailing
-updates) and FP64 iterative refinement with the PCG fallback, against the
+"""Mixed-precision factor (FP32 storage; FP32-MFMA, BF16 hi+lo or scaled FP16
+hi+lo trailing updates) and FP64 iterative refinement with the PCG fallback, against the
 FP64 oracle: the refined solution must reach the FP64 backward error target
 (1e-12) and agree with the oracle's FP64 solution to within what the
 conditioning allows."""
@@ -42,7 +42,15 @@ def test_to_f32_packed_rounds_each_element(hs, dev, N, b, shift):
     assert np.array_equal(L, ref.astype(np.float32))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+# variants: 0 FP32, 1 BF16 hi+lo, 2 scaled FP16 hi+lo
+VARIANTS = [0, 1, 2]
+# ||L L^T - K|| / ||K|| bars: FP32 storage alone gives ~6e-8; the bf16 hi+lo
+# products add ~2^-17 relative, the fp16 ones (22-bit operands) ~2^-23
+RECON_BAR = {0: 2e-6, 1: 2e-5, 2: 2e-6}
+FACTOR_BAR = {0: 1e-4, 1: 1e-3, 2: 1e-4}
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("N,b", [(1024, 128), (1536, 256), (1280, 256), (600, 128), (4096, 256), (3000, 256),
                                  (4096, 128)])
 def test_mixed_factor_reconstructs(hs, dev, oracle, variant, N, b):
@@ -56,14 +64,35 @@ def test_mixed_factor_reconstructs(hs, dev, oracle, variant, N, b):
     K = packed_to_dense(A, N, b)
     err = np.linalg.norm(Ld @ Ld.T - K) / np.linalg.norm(K)
     print("  ||LL^T - K|| / ||K|| = %.2e" % err)
-    # FP32 storage alone gives ~6e-8; the bf16 hi+lo products add ~2^-17
-    assert err < (2e-6 if variant == 0 else 2e-5)
+    assert err < RECON_BAR[variant]
     # the factor agrees with the FP64 oracle's to low-precision accuracy
     Lref = packed_to_dense(oracle.cholesky(A, N, b), N, b, lower_only=True)
-    assert np.max(np.abs(Ld - Lref)) < (1e-4 if variant == 0 else 1e-3) * np.max(np.abs(Lref))
+    assert np.max(np.abs(Ld - Lref)) < FACTOR_BAR[variant] * np.max(np.abs(Lref))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("c", [2.0 ** -30 * 3.0, 1e-7, 1.0, 1e5, 2.0 ** 40 * 5.0])
+@pytest.mark.parametrize("N,b", [(1536, 256), (1280, 128)])
+def test_f16_split_scale_follows_the_matrix(hs, dev, oracle, c, N, b):
+    """The fp16 split scales the panels by a power of two from the largest
+    diagonal entry, so the factor of c K is sqrt(c) times the factor of K to
+    the same relative accuracy whatever c is: well outside the fp16 range
+    (65504 down to 6e-8) on both sides, where an unscaled split would
+    overflow or lose the lo parts to underflow."""
+    print("f16 split scale c=%.3e N=%d b=%d" % (c, N, b), flush=True)
+    A, _ = system(oracle, N, 8, b, 1.0, 1e-1, 5)
+    A = A * c
+    dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
+    assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, 2, 0.0) == 0
+    L = dev.download(dL, packed_len(N, b), dtype=np.float32).astype(np.float64)
+    Ld = packed_to_dense(L, N, b, lower_only=True)
+    K = packed_to_dense(A, N, b)
+    err = np.linalg.norm(Ld @ Ld.T - K) / np.linalg.norm(K)
+    print("  ||LL^T - K|| / ||K|| = %.2e" % err, flush=True)
+    assert np.all(np.isfinite(Ld))
+    assert err < RECON_BAR[2]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("N,b,ell,sn2", [(2048, 128, 1.0, 1e-2), (2048, 256, 2.0, 1e-3)])
 def test_refine_reaches_fp64_backward_error(hs, dev, oracle, variant, N, b, ell, sn2):
     print("refine variant=%d N=%d b=%d" % (variant, N, b), flush=True)
@@ -85,7 +114,7 @@ def test_refine_reaches_fp64_backward_error(hs, dev, oracle, variant, N, b, ell,
     assert np.all(x[N:] == 0.0)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("N,b,ell,sn2", [(2048, 256, 2.0, 1e-3), (3000, 128, 1.0, 1e-2)])
 def test_refine_to_fp64_relative_residual(hs, dev, oracle, variant, N, b, ell, sn2):
     """Equal residual: refinement run to the relative residual the FP64
@@ -142,24 +171,35 @@ def test_pcg_fallback_and_shift(hs, dev, oracle):
         assert fwd <= 10 * cond * 1e-12
 
 
-def test_mixed_reports_bad_pivot(hs, dev):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_mixed_reports_bad_pivot(hs, dev, variant):
     N, b = 512, 128
     K = np.eye(N)
     K[300, 300] = -2.0
     A = dense_to_packed(K, b)
     dA, dL = dev.upload(A), dev.alloc(packed_len(N, b) * 4)
-    assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, 1, 0.0) == 301
+    assert hs.call("hs_cholesky_mixed", dev.h, dA.p, dL.p, N, b, variant, 0.0) == 301
 
 
-@pytest.mark.parametrize("entry,tol", [("hs_syrk_gemm_trailing_f32", 2e-6), ("hs_syrk_gemm_trailing_bf16x3", 2e-5)])
+@pytest.mark.parametrize("entry,tol,scale", [("hs_syrk_gemm_trailing_f32", 2e-6, 1.0),
+                                             ("hs_syrk_gemm_trailing_bf16x3", 2e-5, 1.0),
+                                             ("hs_syrk_gemm_trailing_f16x3", 2e-6, 1.0),
+                                             ("hs_syrk_gemm_trailing_f16x3", 2e-6, 3.0e6),
+                                             ("hs_syrk_gemm_trailing_f16x3", 2e-6, 1.7e-9)])
 @pytest.mark.parametrize("N,b,k", [(1024, 128, 2), (1536, 256, 0), (1280, 256, 3)])
-def test_trailing_step_entries(hs, dev, entry, tol, N, b, k):
+def test_trailing_step_entries(hs, dev, entry, tol, scale, N, b, k):
     """One fused trailing-update step on FP32 storage: A_IJ -= L_Ik L_Jk^T for
     k < J <= I < n (lower triangle of the diagonal blocks), against the same
     update in FP64 on the stored FP32 values; every other entry untouched."""
     n = -(-N // b)
     rng = np.random.default_rng(N + b + k)
-    A = rng.uniform(-1, 1, packed_len(N, b)).astype(np.float32)
+    # scale: panel entries far outside the fp16 range, which the fp16 entry
+    # scales back into it from the panel's largest |entry|; the rest of the
+    # matrix at scale^2, the size of the products it receives
+    A = scale * scale * rng.uniform(-1, 1, packed_len(N, b))
+    p0 = (k * n - k * (k - 1) // 2) * b * b
+    A[p0:p0 + (n - k) * b * b] /= scale
+    A = A.astype(np.float32)
     dA = dev.upload(A)
     assert hs.call(entry, dev.h, dA.p, b, k, n) == 0
     got = dev.download(dA, len(A), dtype=np.float32).astype(np.float64)
@@ -177,8 +217,9 @@ def test_trailing_step_entries(hs, dev, entry, tol, N, b, k):
             if I == J:
                 upd = np.where(np.tril(np.ones((b, b))) > 0, upd, ref[blk(I, J)].reshape(b, b))
             ref[blk(I, J)] = upd.reshape(-1)
-    # |sum over b products of entries <= 1|: the error bar scales with b
-    assert np.max(np.abs(got - ref)) <= tol * b
+    # b products of panel entries <= scale added to entries <= scale^2: the
+    # error bar scales with b scale^2
+    assert np.max(np.abs(got - ref)) <= tol * b * scale * scale
     changed = np.zeros(len(A), dtype=bool)
     for J in range(k + 1, n):
         for I in range(J, n):
