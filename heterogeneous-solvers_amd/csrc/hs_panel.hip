@@ -8,9 +8,10 @@
 //                      update of a panel by the finished columns runs on
 //                      v_mfma_f64_16x16x4f64 with operands loaded straight from
 //                      the block (32 B per lane), the panel is then factored
-//                      with thread t holding row t in registers and one
-//                      barrier per column. It also writes the inverses of the
-//                      16x16 diagonal sub-blocks of the factor for the TRSM.
+//                      with thread t holding row t in registers, one FMA per
+//                      entry and column and one barrier per column. It also
+//                      writes the inverses of the 16x16 diagonal sub-blocks
+//                      of the factor for the TRSM.
 //  trsm_inv_kernel     B <- B L^{-T} for every panel block: one wave owns 16
 //                      rows of one block and solves L X^T = B^T by 16-row block
 //                      substitution entirely in registers. The MFMA output
@@ -33,6 +34,20 @@ namespace {
 
 constexpr int PW = 32;       // POTRF panel width
 constexpr int PP = PW + 1;   // LDS pitch of a panel row
+
+// 1 / sqrt(d) for d > 0 to within an ulp or two: v_rsq_f64 (relative error
+// below 2^-22) and two Newton steps y += y (1 - d y^2) / 2, each squaring the
+// error. Shorter on the dependent path than an IEEE square root followed by
+// an IEEE division.
+__device__ inline double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double r = fma(-(d * y), y, 1.0);
+        y = fma(0.5 * y, r, y);
+    }
+    return y;
+}
 
 // four consecutive stored values as doubles
 __device__ inline void load4(const double* p, double* o) {
@@ -152,22 +167,30 @@ __global__ __launch_bounds__(256) void potrf_rows_kernel(ST* __restrict__ A, i64
         double x[PW];
 #pragma unroll
         for (int c = 0; c < PW; ++c) x[c] = have ? P[t * PP + c] : 0.0;
+        // One column per barrier. Rows c .. PW-1 publish their column-c
+        // value; with s = sqrt(A_cc), the row's multiplier a = L_tc / s
+        // turns L_tc L_jc = a A_jc into one FMA on the published value, and
+        // 1 / s comes from rsqrt_nr. A padding pivot (p + c >= nvalid) is
+        // the square root of the stored diagonal; padding rows keep their
+        // stored values.
 #pragma unroll
         for (int c = 0; c < PW; ++c) {
             double* bcb = bc + (c & 1) * PW;
             if (t >= c && t < PW) bcb[t] = x[c];
             __syncthreads();
             const double d = bcb[c];
-            const double s = sqrt(d);
-            if (t == 0 && p + c < nvalid && !(d > 0.0)) atomicCAS(info, 0, (int)(rowBase + p + c + 1));
+            const bool vc = p + c < nvalid;
+            if (t == 0 && vc && !(d > 0.0)) atomicCAS(info, 0, (int)(rowBase + p + c + 1));
             if (have) {
                 if (t == c) {
-                    x[c] = s;
+                    x[c] = sqrt(d);
                 } else if (t > c && valid) {
-                    x[c] = x[c] / s;
-                    const double inv = 1.0 / s;
+                    const double inv = rsqrt_nr(d);
+                    const double l = x[c] * inv;  // L_tc
+                    const double a = l * inv;
+                    x[c] = l;
 #pragma unroll
-                    for (int cc = c + 1; cc < PW; ++cc) x[cc] -= x[c] * (bcb[cc] * inv);
+                    for (int cc = c + 1; cc < PW; ++cc) x[cc] = fma(-a, bcb[cc], x[cc]);
                 }
             }
         }
@@ -193,33 +216,32 @@ __global__ __launch_bounds__(256) void potrf_rows_kernel(ST* __restrict__ A, i64
             if (p + r >= nvalid) break;
             for (int j = p + PW + tid; j < b; j += 256) B[(i64)(p + r) * b + j] = (ST)0;
         }
-        // (5) inverses of the two 16x16 diagonal sub-blocks of this panel,
-        // W[tile][i][j], column j by thread 16 sb + j; zero past N
-        if (W) {
-            if (t < PW) {
-#pragma unroll
-                for (int c = 0; c < PW; ++c) P[t * PP + c] = x[c];
-            }
-            __syncthreads();
-            if (t < PW) {
-                const int sb = t >> 4, j = t & 15;
-                const int base = 16 * sb;
-                double w[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    double a = 0.0;
-#pragma unroll
-                    for (int k = 0; k < i; ++k) a += P[(base + i) * PP + base + k] * w[k];
-                    const double num = (i == j) ? 1.0 : -a;
-                    w[i] = (i < j) ? 0.0 : num / P[(base + i) * PP + base + i];
-                }
-                double* Wt = W + (i64)(p / 16 + sb) * 256;
-                const bool colValid = p + base + j < nvalid;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) Wt[i * 16 + j] = (colValid && p + base + i < nvalid) ? w[i] : 0.0;
-            }
-        }
         __syncthreads();
+    }
+    // (5) inverses of the 16x16 diagonal sub-blocks of the stored factor,
+    // W[tile][i][j], one column j of tile tb per thread 16 tb + j, all tiles
+    // at once (the stores of every panel are visible to the work-group after
+    // the barrier above); zero past N. The reciprocals of the diagonal come
+    // first, all independent, so that no division sits on the substitution's
+    // dependent path.
+    if (W && tid < 16 * (b / 16)) {
+        const int tb = tid >> 4, j = tid & 15, base = 16 * tb;
+        const ST* Ls = B + (i64)base * b + base;
+        double w[16], rd[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) rd[i] = 1.0 / (double)Ls[(i64)i * b + i];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) a += (double)Ls[(i64)i * b + k] * w[k];
+            const double num = (i == j) ? 1.0 : -a;
+            w[i] = (i < j) ? 0.0 : num * rd[i];
+        }
+        double* Wt = W + (i64)tb * 256;
+        const bool colValid = base + j < nvalid;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Wt[i * 16 + j] = (colValid && base + i < nvalid) ? w[i] : 0.0;
     }
 }
 
