@@ -724,34 +724,35 @@ __global__ __launch_bounds__(256) void split_panels_f16_kernel(const float* __re
 
 // The fp16 split's scale. Every entry of a Cholesky factor of A satisfies
 // |L_ij| <= sqrt(A_ii), so one scale serves all of a factorisation's panels:
-// hilo_diag_max_kernel takes the largest diagonal entry of the FP32-stored
-// matrix (rows below N; single work-group), hilo_abs_max_kernel the largest
-// |entry| of one panel (block column k, rows k+1 .. n-1) for the per-step
-// entry; both leave a non-negative bound in sc[2] (as its bits: for
-// non-negative floats the unsigned order is the numeric order, so work-groups
-// combine by atomicMax). hilo_scale_finish_kernel then writes s = 2^(15 - E)
+// hilo_diag_max_kernel takes sqrt of the largest diagonal entry of the
+// FP32-stored matrix (rows below N; one work-group per diagonal block),
+// hilo_abs_max_kernel the largest |entry| of one panel (block column k, rows
+// k+1 .. n-1) for the per-step entry; both leave a non-negative bound in
+// sc[2] (as its bits: for non-negative floats the unsigned order is the
+// numeric order, so work-groups combine by atomicMax; a negative or NaN
+// diagonal counts as 0, and the factorisation reports its pivot). hilo_scale_finish_kernel then writes s = 2^(15 - E)
 // with bound = f 2^E, f in [0.5, 1), so that s * bound < 2^15 (the fp16
 // range ends at 65504), and sc[1] = 1 / s^2. The exponent shift is clamped to
 // [-60, 60] so that 1 / s^2 stays a normal FP32 number.
+__global__ void hilo_zero_kernel(float* __restrict__ sc) {
+    if (threadIdx.x < 4) sc[threadIdx.x] = 0.0f;
+}
+
+// work-group J: the diagonal of block J (after hilo_zero_kernel)
 __global__ __launch_bounds__(256) void hilo_diag_max_kernel(const float* __restrict__ L, int b, int n, i64 N,
                                                             float* __restrict__ sc) {
     __shared__ float red[256];
+    const int J = blockIdx.x;
+    const float* D = L + hs_block_id(J, J, n) * (i64)b * b;
     float mx = 0.0f;
-    for (i64 i = threadIdx.x; i < N; i += 256) {
-        const i64 J = i / b, r = i % b;
-        mx = fmaxf(mx, L[hs_block_id(J, J, n) * (i64)b * b + r * b + r]);
-    }
+    for (int r = threadIdx.x; r < b && (i64)J * b + r < N; r += 256) mx = fmaxf(mx, D[(i64)r * b + r]);
     red[threadIdx.x] = mx;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) sc[2] = sqrtf(red[0]);
-}
-
-__global__ void hilo_zero_kernel(float* __restrict__ sc) {
-    if (threadIdx.x < 4) sc[threadIdx.x] = 0.0f;
+    if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned*>(sc + 2), __float_as_uint(sqrtf(red[0])));
 }
 
 __global__ __launch_bounds__(256) void hilo_abs_max_kernel(const float* __restrict__ A, int b, int n, int k,
@@ -871,7 +872,9 @@ int hs_launch_trailing_mixed(hs_handle_t h, hipStream_t stream, float* A, int b,
 // The fp16 split's scale for a factorisation of the FP32-stored matrix L
 // (from its largest diagonal entry), into h->hilo_scale on h->stream.
 static int hilo_scale_for_factor(hs_handle_t h, const float* L, int b, int n, i64 N) {
-    hilo_diag_max_kernel<<<1, 256, 0, h->stream>>>(L, b, n, N, h->hilo_scale);
+    hilo_zero_kernel<<<1, 64, 0, h->stream>>>(h->hilo_scale);
+    HS_LAUNCH_CHECK();
+    hilo_diag_max_kernel<<<dim3((unsigned)n), 256, 0, h->stream>>>(L, b, n, N, h->hilo_scale);
     HS_LAUNCH_CHECK();
     hilo_scale_finish_kernel<<<1, 64, 0, h->stream>>>(h->hilo_scale);
     HS_LAUNCH_CHECK();
