@@ -255,12 +255,9 @@ static int launch_ordered(hs_handle_t h, const double* A, const double* x, doubl
     const int bands = (b + RB - 1) / RB;
     // two CH x (RB + 1) tiles: 66 KB at CH = 32
     const size_t lds = sizeof(double) * 2 * CH * (RB + 1);
-    static bool raised = false;
-    if (!raised) {
-        HS_HIP(hipFuncSetAttribute((const void*)symv_ordered_kernel<CH, FMA, FULL>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 2 * CH * 129)));
-        raised = true;
-    }
+    // set per launch: the attribute belongs to the device current at the call
+    HS_HIP(hipFuncSetAttribute((const void*)symv_ordered_kernel<CH, FMA, FULL>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 2 * CH * 129)));
     const int T = std::max(64, (RB + 63) / 64 * 64);
     symv_ordered_kernel<CH, FMA, FULL><<<dim3((unsigned)(n * bands)), dim3(T), lds, h->stream>>>(A, x, y, n, b, RB,
                                                                                               stop, it);
