@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # counters read 1 GiB for a copy kernel of 1 GiB read + 1 GiB written
 # (bench.py --copy-calibration); four-panel steps, 62 launches. Summary and
 # commands: profiles/trailing_mfma_f64_cholesky65536_hbm_traffic.md
-TRAILING_TRAFFIC_BYTES = 26.23e9
+TRAILING_TRAFFIC_BYTES = 23.61e9
 
 
 def _load(name, path):

@@ -162,12 +162,26 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, in
 }
 
 // Work-group order of a trailing-update launch: band order (bit 0) and XCD
-// chunking (bit 1), bands of 4 block rows. The FP64 update ran equally fast
-// with and without them (it is bound by MFMA issue, not by its 1.9-2.5 TB/s
-// of memory traffic); both stay on.
-inline void hs_trail_order(int* bits, int* band) {
-    *bits = 3;
-    *band = 4;
+// chunking (bit 1). The FP64 update ran equally fast with and without them
+// (it is bound by MFMA issue, not by its 1.9-2.5 TB/s of memory traffic); both
+// stay on for the L2 re-reads they save. Band height: 3 block rows where a
+// block pair has 4 or more tiles (128-tile FP64 update on b = 256: HBM
+// reads 20.7 / 18.6 / 18.4 / 24.8 / 26.9 GB per rest-update launch at heights
+// 4 / 2 / 3 / 6 / 8, 25.4 GB without chunking; kernel time the same), 4 where
+// one tile covers a block (256-tile mixed updates: height 2 ran 1% slower).
+// HS_TRAIL_BAND = 1..16 and HS_TRAIL_ORDER = 0..3 override them.
+inline void hs_trail_order(i64 tilesPerBlock, int* bits, int* band) {
+    static int b = -1, o = -1;
+    if (b < 0) {
+        const char* e = std::getenv("HS_TRAIL_BAND");
+        const int w = e ? std::atoi(e) : 0;
+        b = (w >= 1 && w <= 16) ? w : 0;
+        const char* f = std::getenv("HS_TRAIL_ORDER");
+        const int v = f ? std::atoi(f) : -1;
+        o = (v >= 0 && v <= 3) ? v : 3;
+    }
+    *bits = o;
+    *band = b > 0 ? b : (tilesPerBlock >= 4 ? 3 : 4);
 }
 
 // Fills the pair map of a trailing-update launch with tilesPerBlock work-
@@ -182,7 +196,7 @@ inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int n
     pm->nc = (mode == 6) ? std::max(1, std::min(rowCount, n - rs)) : 1;
     if (pairs <= 0) return 0;
     int bits, band;
-    hs_trail_order(&bits, &band);
+    hs_trail_order(tilesPerBlock, &bits, &band);
     if ((bits & 1) && (mode == 4 || (mode == 3 && off == 0))) {
         pm->m = (mode == 4) ? n - rs : rowCount;
         pm->band = band;
