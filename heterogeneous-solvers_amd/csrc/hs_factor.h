@@ -163,14 +163,17 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, in
 
 // Work-group order of a trailing-update launch: band order (bit 0) and XCD
 // chunking (bit 1). The FP64 update ran equally fast with and without them
-// (it is bound by MFMA issue, not by its 1.9-2.5 TB/s of memory traffic); both
-// stay on for the L2 re-reads they save. Band height: 3 block rows where a
-// block pair has 4 or more tiles (128-tile FP64 update on b = 256: HBM
-// reads 20.7 / 18.6 / 18.4 / 24.8 / 26.9 GB per rest-update launch at heights
-// 4 / 2 / 3 / 6 / 8, 25.4 GB without chunking; kernel time the same), 4 where
-// one tile covers a block (256-tile mixed updates: height 2 ran 1% slower).
-// HS_TRAIL_BAND = 1..16 and HS_TRAIL_ORDER = 0..3 override them.
-inline void hs_trail_order(i64 tilesPerBlock, int* bits, int* band) {
+// (it is bound by MFMA issue, not by its 1-2.5 TB/s of memory traffic); both
+// stay on for the L2 re-reads they save. Band height where a block pair has 4
+// or more tiles (128-tile FP64 update on b = 256), from the HBM reads per
+// rest-update launch (FETCH_SIZE) at N = 65536, kernel time the same for all:
+//   4 panels: 20.7 / 18.6 / 18.4 / 24.8 / 26.9 GB at heights 4 / 2 / 3 / 6 / 8
+//             (25.4 GB at height 4 without chunking)
+//   6 panels: 45.2 / 32.2 / 35.5 / 41.2 GB at heights 1 / 2 / 3 / 4
+// so about 6 MB of a band's operand rows: height 3 up to 4 panels, 2 beyond.
+// Height 4 where one tile covers a block (256-tile mixed updates: height 2
+// ran 1% slower). HS_TRAIL_BAND = 1..16 and HS_TRAIL_ORDER = 0..3 override.
+inline void hs_trail_order(i64 tilesPerBlock, int np, int* bits, int* band) {
     static int b = -1, o = -1;
     if (b < 0) {
         const char* e = std::getenv("HS_TRAIL_BAND");
@@ -181,7 +184,7 @@ inline void hs_trail_order(i64 tilesPerBlock, int* bits, int* band) {
         o = (v >= 0 && v <= 3) ? v : 3;
     }
     *bits = o;
-    *band = b > 0 ? b : (tilesPerBlock >= 4 ? 3 : 4);
+    *band = b > 0 ? b : (tilesPerBlock >= 4 ? (np >= 5 ? 2 : 3) : 4);
 }
 
 // Fills the pair map of a trailing-update launch with tilesPerBlock work-
@@ -196,7 +199,7 @@ inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int n
     pm->nc = (mode == 6) ? std::max(1, std::min(rowCount, n - rs)) : 1;
     if (pairs <= 0) return 0;
     int bits, band;
-    hs_trail_order(tilesPerBlock, &bits, &band);
+    hs_trail_order(tilesPerBlock, np, &bits, &band);
     if ((bits & 1) && (mode == 4 || (mode == 3 && off == 0))) {
         pm->m = (mode == 4) ? n - rs : rowCount;
         pm->band = band;
@@ -281,15 +284,18 @@ struct PhaseTimer {
     }
 };
 
-// Panels per step of the look-ahead schedule: 4 (trailing updates of depth
-// 4b; measured at N = 65536, b = 256: 1569 / 1526 / 1506 ms for 2 / 3 / 4),
-// HS_PANELS = 1..4 to choose another, and never more than n/2.
+// Panels per step of the look-ahead schedule: 6 (trailing updates of depth
+// 6b). Measured FP64 factor at N = 65536, b = 256: 1416-1420 / 1408 / 1404-1406
+// / 1411 ms for 4 / 5 / 6 / 8 (deeper launches amortise a work-group's
+// prologue and epilogue; the longer panel chain costs at 8); GP NLL + gradient
+// at N = 16384, b = 128 the same for 4 and 6. HS_PANELS = 1..8 chooses
+// another depth, never more than n/2.
 inline int hs_schedule_panels(int n) {
     static int v = -1;
     if (v < 0) {
         const char* e = std::getenv("HS_PANELS");
         const int w = e ? std::atoi(e) : 0;
-        v = (w >= 1 && w <= 4) ? w : 4;
+        v = (w >= 1 && w <= 8) ? w : 6;
     }
     return std::max(1, std::min(v, n / 2));
 }
