@@ -199,16 +199,20 @@ def test_analysis_catches_a_missing_wait(planes):
 
 
 def test_dry_run_matches_the_factors_panel_counts():
-    """The rest-update launches of the recorded schedule are the ones the
-    bench's roofline counts (hs_cholesky_schedule_f64)."""
+    """The rest-update launches of the recorded schedule at the library's
+    default depth (panels = 0) are the ones the bench's roofline counts
+    (hs_cholesky_schedule_f64): 41 at N = 65536, b = 256 (six panels per
+    step)."""
     import ctypes as C
 
     from hs_testlib import load_hs
 
     hs = load_hs()
     for n, b in ((256, 256), (32, 128), (7, 64)):
-        ops = schedule(n, max(1, min(4, n // 2)))
+        ops = schedule(n, 0)
         rest = [op for op in ops if op[1:2] == ('trail',) and op[2][1] == 4]
         cnt, fl = C.c_int64(), C.c_double()
         hs.call("hs_cholesky_schedule_f64", n * b, b, C.byref(cnt), C.byref(fl))
         assert len(rest) == cnt.value
+        if n == 256:
+            assert cnt.value == 41
