@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # average over the look-ahead rest-update launches at N = 65536, b = 256):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors that make the
 # counters read 1 GiB for a copy kernel of 1 GiB read + 1 GiB written
-# (bench.py --copy-calibration); six-panel steps, 41 launches. Summary and
+# (bench.py --copy-calibration); four-panel steps, 62 launches. Summary and
 # commands: profiles/trailing_mfma_f64_cholesky65536_hbm_traffic.md
-TRAILING_TRAFFIC_BYTES = 38.19e9
+TRAILING_TRAFFIC_BYTES = 23.58e9
 
 
 def _load(name, path):
@@ -643,7 +643,7 @@ def main():
         "roofline": {"kernel": "trailing_mfma_f64 (fused SYRK+GEMM trailing update)", "bound": "mfma",
                      "achieved": up["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": up["tflops"] / FP64_PEAK_TFLOPS,
-                     "traffic": (TRAILING_TRAFFIC_BYTES if (ch["N"], ch["b"], up["launches"]) == (65536, 256, 41)
+                     "traffic": (TRAILING_TRAFFIC_BYTES if (ch["N"], ch["b"], up["launches"]) == (65536, 256, 62)
                                  else None),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "flops_per_launch": up["flops_per_launch"], "avg_launch_ms": up["avg_launch_ms"],

@@ -284,18 +284,21 @@ struct PhaseTimer {
     }
 };
 
-// Panels per step of the look-ahead schedule: 6 (trailing updates of depth
-// 6b). Measured FP64 factor at N = 65536, b = 256: 1416-1420 / 1408 / 1404-1406
-// / 1411 ms for 4 / 5 / 6 / 8 (deeper launches amortise a work-group's
-// prologue and epilogue; the longer panel chain costs at 8); GP NLL + gradient
-// at N = 16384, b = 128 the same for 4 and 6. HS_PANELS = 1..8 chooses
-// another depth, never more than n/2.
+// Panels per step of the look-ahead schedule: 4 (trailing updates of depth
+// 4b). Measured FP64 factor at N = 65536, b = 256: 1569 / 1526 / 1506 ms for
+// 2 / 3 / 4 with an earlier update kernel; with the current one 1416-1420 /
+// 1408 / 1404-1406 / 1411 ms for 4 / 5 / 6 / 8. Six panels save 0.8% of the
+// factor but move work to the panel stream's column updates (the rest update
+// then runs at 61.3 instead of 62.4 TFLOP/s) and re-read more operand tiles
+// (38.2 instead of 23.6 GB of HBM traffic per launch against 11.2 and 11.3
+// algorithmic), so four stay. GP NLL + gradient at N = 16384, b = 128: the
+// same for 4 and 6. HS_PANELS = 1..8 chooses another depth, never more than n/2.
 inline int hs_schedule_panels(int n) {
     static int v = -1;
     if (v < 0) {
         const char* e = std::getenv("HS_PANELS");
         const int w = e ? std::atoi(e) : 0;
-        v = (w >= 1 && w <= 8) ? w : 6;
+        v = (w >= 1 && w <= 8) ? w : 4;
     }
     return std::max(1, std::min(v, n / 2));
 }

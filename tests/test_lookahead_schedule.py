@@ -201,7 +201,7 @@ def test_analysis_catches_a_missing_wait(planes):
 def test_dry_run_matches_the_factors_panel_counts():
     """The rest-update launches of the recorded schedule at the library's
     default depth (panels = 0) are the ones the bench's roofline counts
-    (hs_cholesky_schedule_f64): 41 at N = 65536, b = 256 (six panels per
+    (hs_cholesky_schedule_f64): 62 at N = 65536, b = 256 (four panels per
     step)."""
     import ctypes as C
 
@@ -215,4 +215,4 @@ def test_dry_run_matches_the_factors_panel_counts():
         hs.call("hs_cholesky_schedule_f64", n * b, b, C.byref(cnt), C.byref(fl))
         assert len(rest) == cnt.value
         if n == 256:
-            assert cnt.value == 41
+            assert cnt.value == 62
