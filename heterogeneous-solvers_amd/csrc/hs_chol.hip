@@ -218,7 +218,7 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
 // three times that beside full-CU update work-groups. Half-CU costs the
 // update a fifth of its rate; measured best (GP NLL at N = 16384, b = 128:
 // NLL 39.2 -> 37.6 ms) with the bound b^3 m^2 / 60 TFLOP/s < 2.5 * 1.1 us * b,
-// i.e. m <= 99 at b = 128 and m <= 50 at b = 256 (no change at N = 65536).
+// i.e. m <= 100 at b = 128 and m <= 50 at b = 256 (no change at N = 65536).
 // HS_HALF_CU_ROWS sets another bound (0 turns it off).
 static int hs_half_cu_rows(int b) {
     static int env = -2;
