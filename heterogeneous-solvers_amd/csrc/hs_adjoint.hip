@@ -355,14 +355,28 @@ __global__ __launch_bounds__(256) void adj_symm128_kernel(const double* __restri
                 __builtin_nontemporal_store(acc[i][j][v], Wt + (i64)(wm0 + i * 16 + lq + 4 * v) * b + wn0 + j * 16 + lr);
 }
 
+// s + sum_{c < nch} W[c * stride] in order c = 0, 1, ..., with the loads of
+// 8 consecutive c issued together (one load at a time left the sums of the
+// reverse sweep latency-bound at 1-2 TB/s). The partials are read once.
+__device__ inline double sum_slots(const double* __restrict__ W, i64 stride, int nch) {
+    double s = 0.0;
+    int c = 0;
+    for (; c + 8 <= nch; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(W + (i64)(c + u) * stride);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < nch; ++c) s += __builtin_nontemporal_load(W + (i64)c * stride);
+    return s;
+}
+
 // dst[e] -= sum_{c < nch} W[c * stride + e], e < count (fixed order)
 __global__ void adj_sub_partials_kernel(double* __restrict__ dst, const double* __restrict__ W, i64 count, i64 stride,
                                         int nch) {
-    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (i64)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < nch; ++c) s += W[(i64)c * stride + e];
-        dst[e] -= s;
-    }
+    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (i64)gridDim.x * blockDim.x)
+        dst[e] -= sum_slots(W + e, stride, nch);
 }
 
 __global__ void adj_copy_kernel(double* __restrict__ dst, const double* __restrict__ src, i64 count) {
@@ -370,15 +384,18 @@ __global__ void adj_copy_kernel(double* __restrict__ dst, const double* __restri
         dst[e] = src[e];
 }
 
-// Dbar (b x b, lower) -= tril(sum_c Wg[c]) with Wg[c] = (Abar_panel^T P) partials
+// Dbar (b x b, lower) -= tril(sum_c Wg[c]) with Wg[c] = (Abar_panel^T P)
+// partials, summed in order c = 0 .. nch-1; thread t takes the t-th element
+// of the lower triangle in row order (row r starts at r (r + 1) / 2)
 __global__ void adj_gram_sub_kernel(double* __restrict__ Dbar, const double* __restrict__ Wg, int b, int nch) {
-    const i64 bb = (i64)b * b;
-    for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < bb; e += (i64)gridDim.x * blockDim.x) {
-        const int r = (int)(e / b), c = (int)(e % b);
-        if (c > r) continue;
-        double s = 0.0;
-        for (int q = 0; q < nch; ++q) s += Wg[q * bb + e];
-        Dbar[e] -= s;
+    const i64 bb = (i64)b * b, tri = (i64)b * (b + 1) / 2;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < tri; t += (i64)gridDim.x * blockDim.x) {
+        int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((i64)r * (r + 1) / 2 > t) --r;
+        while ((i64)(r + 1) * (r + 2) / 2 <= t) ++r;
+        const int c = (int)(t - (i64)r * (r + 1) / 2);
+        const i64 e = (i64)r * b + c;
+        Dbar[e] -= sum_slots(Wg + e, bb, nch);
     }
 }
 
@@ -647,7 +664,7 @@ int reverse_tail(hs_handle_t h, const double* L, double* Ab, int b, int n, int k
         // single batched launch, summed in a fixed order
         GemmArgs g2{Pbar, bb, b, 1, P, bb, b, ws + w.wg, bb, b, b, tilesN, tiles, 0};
         if ((rc = launch_gemm(h, g2, m, tilesN))) return rc;
-        adj_gram_sub_kernel<<<dim3(grid_for(bb)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, m);
+        adj_gram_sub_kernel<<<dim3(grid_for((i64)b * (b + 1) / 2)), dim3(256), 0, h->stream>>>(Akk, ws + w.wg, b, m);
         HS_LAUNCH_CHECK();
     }
     // POTRF reverse on block k
