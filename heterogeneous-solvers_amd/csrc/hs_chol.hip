@@ -221,11 +221,10 @@ int hs_launch_trailing(hs_handle_t h, double* A, int b, int k, int rowStart, int
 // i.e. m <= 100 at b = 128 and m <= 50 at b = 256 (no change at N = 65536).
 // HS_HALF_CU_ROWS sets another bound (0 turns it off).
 static int hs_half_cu_rows(int b) {
-    static int env = -2;
-    if (env == -2) {
+    static const int env = [] {
         const char* e = std::getenv("HS_HALF_CU_ROWS");
-        env = e ? std::max(0, std::atoi(e)) : -1;
-    }
+        return e ? std::max(0, std::atoi(e)) : -1;
+    }();
     if (env >= 0) return env;
     if (b % 128 != 0) return 0;
     const double m2 = 60e12 * 2.5 * 1.1e-6 * b / ((double)b * b * b);

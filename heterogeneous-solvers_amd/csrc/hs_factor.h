@@ -174,15 +174,17 @@ inline void hs_pair_range(int k, int rowStart, int rowCount, int n, int mode, in
 // Height 4 where one tile covers a block (256-tile mixed updates: height 2
 // ran 1% slower). HS_TRAIL_BAND = 1..16 and HS_TRAIL_ORDER = 0..3 override.
 inline void hs_trail_order(i64 tilesPerBlock, int np, int* bits, int* band) {
-    static int b = -1, o = -1;
-    if (b < 0) {
+    // read once per process (thread-safe static initialisation)
+    static const int b = [] {
         const char* e = std::getenv("HS_TRAIL_BAND");
         const int w = e ? std::atoi(e) : 0;
-        b = (w >= 1 && w <= 16) ? w : 0;
+        return (w >= 1 && w <= 16) ? w : 0;
+    }();
+    static const int o = [] {
         const char* f = std::getenv("HS_TRAIL_ORDER");
         const int v = f ? std::atoi(f) : -1;
-        o = (v >= 0 && v <= 3) ? v : 3;
-    }
+        return (v >= 0 && v <= 3) ? v : 3;
+    }();
     *bits = o;
     *band = b > 0 ? b : (tilesPerBlock >= 4 ? (np >= 5 ? 2 : 3) : 4);
 }
@@ -294,12 +296,11 @@ struct PhaseTimer {
 // algorithmic), so four stay. GP NLL + gradient at N = 16384, b = 128: the
 // same for 4 and 6. HS_PANELS = 1..8 chooses another depth, never more than n/2.
 inline int hs_schedule_panels(int n) {
-    static int v = -1;
-    if (v < 0) {
+    static const int v = [] {
         const char* e = std::getenv("HS_PANELS");
         const int w = e ? std::atoi(e) : 0;
-        v = (w >= 1 && w <= 8) ? w : 4;
-    }
+        return (w >= 1 && w <= 8) ? w : 4;
+    }();
     return std::max(1, std::min(v, n / 2));
 }
 

@@ -956,12 +956,11 @@ static int split_panels(hipStream_t stream, const float* L, int b, int n, int k,
 // / 806 ms. HS_MIXED_PANELS = 1..8 chooses another depth.
 static int mixed_panels(int n, int variant) {
     (void)variant;
-    static int v = -1;
-    if (v < 0) {
+    static const int v = [] {
         const char* e = std::getenv("HS_MIXED_PANELS");
         const int w = e ? std::atoi(e) : 0;
-        v = (w >= 1 && w <= 8) ? w : 6;
-    }
+        return (w >= 1 && w <= 8) ? w : 6;
+    }();
     return std::max(1, std::min(v, n / 2));
 }
 
