@@ -519,6 +519,39 @@ extern "C" int hs_lookahead_dry_run(int n, int panels, int planes, int64_t* ops,
     return HS_OK;
 }
 
+// The work-group map of one trailing-update launch, replayed on the host with
+// the kernels' own decode: for work-group g, out[3g .. 3g+2] = (I, J, tile),
+// or (-1, -1, -1) for an idle work-group. band <= 0 and order < 0 take the
+// library's defaults for this launch. *count receives the grid size; at most
+// `capacity` work-groups are written.
+extern "C" int hs_trail_map_dry_run(int n, int k, int rowStart, int rowCount, int mode, int np, int tilesPerBlock,
+                                    int band, int order, int32_t* out, int64_t capacity, int64_t* count) {
+    if (n <= 0) return -1;
+    if (mode < 1 || mode > 6) return -5;
+    if (np < 1) return -6;
+    if (tilesPerBlock < 1) return -7;
+    if (band > 16) return -8;
+    if (order > 3) return -9;
+    if (!out && capacity > 0) return -10;
+    if (!count) return -12;
+    int bits, h;
+    hs_trail_order(tilesPerBlock, np, &bits, &h);
+    if (band > 0) h = band;
+    if (order >= 0) bits = order;
+    PairMap pm;
+    const i64 grid = hs_pair_map_order(k, rowStart, rowCount, n, mode, np, tilesPerBlock, bits, h, &pm);
+    *count = grid;
+    const i64 w = std::min<i64>(grid, capacity);
+    for (i64 g = 0; g < w; ++g) {
+        int I = -1, J = -1, tile = -1;
+        if (!tile_of_g(pm, tilesPerBlock, g, &I, &J, &tile)) I = J = tile = -1;
+        out[3 * g] = I;
+        out[3 * g + 1] = J;
+        out[3 * g + 2] = tile;
+    }
+    return HS_OK;
+}
+
 // Work of the rest-update launches of the look-ahead schedule the factorisations
 // use for N and b (b a multiple of 64): their count and algorithmic flops.
 extern "C" int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops) {

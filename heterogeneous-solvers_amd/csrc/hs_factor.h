@@ -34,7 +34,7 @@ struct PairMap {
     i64 total = 0, per = 0;
 };
 
-__device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
+__host__ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
     if (pm.mode == 1) {
         *I = pm.rowStart + (int)q;
         *J = *I;
@@ -79,7 +79,7 @@ __device__ inline void pair_of(const PairMap& pm, i64 q, int* I, int* J) {
 // band's rows below the diagonal. Consecutive work-groups then share the
 // B operand L_Jk while the band's H A operands stay in the XCD's L2, so an
 // operand block is fetched once per band instead of once per row.
-__device__ inline void band_decode(i64 q, int m, int H, int* pi, int* pj) {
+__host__ __device__ inline void band_decode(i64 q, int m, int H, int* pi, int* pj) {
     auto S = [&](i64 B) { return (i64)H * H * B * (B - 1) / 2 + B * (i64)H * (H + 1) / 2; };
     int lo = 0, hi = (m + H - 1) / H - 1;
     while (lo < hi) {
@@ -89,7 +89,7 @@ __device__ inline void band_decode(i64 q, int m, int H, int* pi, int* pj) {
     }
     i64 p = q - S(lo);
     const int r0 = lo * H;
-    const int hp = min(H, m - r0);
+    const int hp = (m - r0 < H) ? m - r0 : H;
     const i64 rect = (i64)hp * r0;
     if (p < rect) {
         *pj = (int)(p / hp);
@@ -106,10 +106,9 @@ __device__ inline void band_decode(i64 q, int m, int H, int* pi, int* pj) {
     *pi = r0 + t + (int)p;
 }
 
-// Block pair (I, J) and tile index of this work-group; false for the idle
+// Block pair (I, J) and tile index of work-group g; false for the idle
 // work-groups that pad an XCD-chunked launch to a multiple of 8.
-__device__ inline bool tile_of(const PairMap& pm, int tilesPerBlock, int* I, int* J, int* tile) {
-    i64 g = blockIdx.x;
+__host__ __device__ inline bool tile_of_g(const PairMap& pm, int tilesPerBlock, i64 g, int* I, int* J, int* tile) {
     if (pm.per > 0) {
         g = (g % 8) * pm.per + g / 8;
         if (g >= pm.total) return false;
@@ -125,6 +124,10 @@ __device__ inline bool tile_of(const PairMap& pm, int tilesPerBlock, int* I, int
         pair_of(pm, q, I, J);
     }
     return true;
+}
+
+__device__ inline bool tile_of(const PairMap& pm, int tilesPerBlock, int* I, int* J, int* tile) {
+    return tile_of_g(pm, tilesPerBlock, blockIdx.x, I, J, tile);
 }
 
 // Number of block pairs and the pair offset of a launch (see PairMap).
@@ -190,8 +193,10 @@ inline void hs_trail_order(i64 tilesPerBlock, int np, int* bits, int* band) {
 }
 
 // Fills the pair map of a trailing-update launch with tilesPerBlock work-
-// groups per block pair; returns the grid size (0: nothing to do).
-inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int np, i64 tilesPerBlock, PairMap* pm) {
+// groups per block pair, in the work-group order `bits` with bands of `band`
+// block rows; returns the grid size (0: nothing to do).
+inline i64 hs_pair_map_order(int k, int rowStart, int rowCount, int n, int mode, int np, i64 tilesPerBlock, int bits,
+                             int band, PairMap* pm) {
     i64 pairs, off;
     int rs;
     hs_pair_range(k, rowStart, rowCount, n, mode, np, &pairs, &off, &rs);
@@ -200,8 +205,6 @@ inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int n
     pm->np = np;
     pm->nc = (mode == 6) ? std::max(1, std::min(rowCount, n - rs)) : 1;
     if (pairs <= 0) return 0;
-    int bits, band;
-    hs_trail_order(tilesPerBlock, np, &bits, &band);
     if ((bits & 1) && (mode == 4 || (mode == 3 && off == 0))) {
         pm->m = (mode == 4) ? n - rs : rowCount;
         pm->band = band;
@@ -213,6 +216,13 @@ inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int n
         return 8 * pm->per;
     }
     return total;
+}
+
+// hs_pair_map_order in the launch order of hs_trail_order
+inline i64 hs_pair_map(int k, int rowStart, int rowCount, int n, int mode, int np, i64 tilesPerBlock, PairMap* pm) {
+    int bits, band;
+    hs_trail_order(tilesPerBlock, np, &bits, &band);
+    return hs_pair_map_order(k, rowStart, rowCount, n, mode, np, tilesPerBlock, bits, band, pm);
 }
 
 // Runs launches that use h->stream on another stream of the same handle.

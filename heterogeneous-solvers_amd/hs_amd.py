@@ -72,6 +72,7 @@ _SIGS = {
     "hs_tri_solve_prepared_f64": [_P, _P, _P, _P, _L, _I],
     "hs_cholesky_schedule_f64": [_L, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_lookahead_dry_run": [_I, _I, _I, _P, _L, C.POINTER(_L)],
+    "hs_trail_map_dry_run": [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _L, C.POINTER(_L)],
     "hs_cholesky_mixed_schedule": [_L, _I, _I, C.POINTER(_L), C.POINTER(_D)],
     "hs_read_sum_f64": [_P, _P, _L, _P],
     "hs_cg_f64": [_P, _P, _P, _P, _L, _I, _D, _L, C.POINTER(_L), C.POINTER(_D), C.POINTER(_D)],

@@ -213,6 +213,21 @@ int hs_cholesky_f64_trace(hs_handle_t h, double* A, int64_t N, int b, double* po
 int hs_lookahead_dry_run(int n, int panels, int planes, int64_t* ops, int64_t capacity, int64_t* count);
 int hs_cholesky_schedule_f64(int64_t N, int b, int64_t* rest_launches, double* rest_flops);
 
+/* The work-group map of one fused trailing-update launch, replayed on the host
+ * with the kernels' own decode (no device needed): step k, rows rowStart ..
+ * rowStart+rowCount-1, mode 1-6 (1 diagonal blocks, 2 off-diagonal, 3 both,
+ * 4 look-ahead rest of the triangle, 5 one look-ahead column, 6 look-ahead
+ * columns), np panels, tilesPerBlock work-groups per block pair; band (block
+ * rows, 1..16) and order (bit 0 bands, bit 1 one contiguous stretch per XCD)
+ * override the library's choice when band > 0 and order >= 0. For work-group
+ * g, out[3g .. 3g+2] = (I, J, tile), or -1 three times for the idle
+ * work-groups that pad a launch to a multiple of 8; *count = grid size, at
+ * most `capacity` work-groups written. No reference counterpart: the
+ * reference decodes one (I, J) per work-group with its triangular-number
+ * square root (MatrixMatrixOperations.cpp:698-710). */
+int hs_trail_map_dry_run(int n, int k, int rowStart, int rowCount, int mode, int np, int tilesPerBlock, int band,
+                         int order, int32_t* out, int64_t capacity, int64_t* count);
+
 /* The same for the look-ahead schedule of hs_cholesky_mixed with this variant
  * (b = 128 or 256), whose steps may apply more panels than the FP64 one.
  * Measurement helper; no reference counterpart. */
