@@ -953,7 +953,9 @@ static int split_panels(hipStream_t stream, const float* L, int b, int n, int k,
 // alone: 1165 / 1224 / 1257 TFLOP/s over the three products for 4 / 6 / 8
 // panels), against a longer panel chain per step. Factor at N = 65536,
 // b = 256 for 4 / 6 / 8 panels: BF16 hi+lo 324 / 318 / 322 ms, FP32 808 / 797
-// / 806 ms. HS_MIXED_PANELS = 1..8 chooses another depth.
+// / 806 ms; with the current kernels BF16 hi+lo 297 / 290 / 291, scaled FP16
+// hi+lo 305 / 299 / 298, FP32 744 / 732 / 728 ms. HS_MIXED_PANELS = 1..8
+// chooses another depth.
 static int mixed_panels(int n, int variant) {
     (void)variant;
     static const int v = [] {
