@@ -153,6 +153,46 @@ __global__ __launch_bounds__(ATHREADS) void adj_gemm_kernel(GemmArgs g) {
     });
 }
 
+// adj_gemm_kernel (row-k A, mode 0) with every column tile of a 64-row strip
+// in one work-group: each tile accumulates over k in the same order, and the
+// whole A strip is read before any C element is written, so C may be A (the
+// panel adjoint times D^{-1} in place, without a copy back).
+__global__ __launch_bounds__(ATHREADS) void adj_gemm_strip_kernel(GemmArgs g) {
+    constexpr int MAXN = 4;  // b <= 256
+    __shared__ double As[AKT * ALDP];
+    __shared__ double Bs[AKT * ALDP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tilesM = g.tilesPer / g.tilesN;
+    const int bi = blockIdx.x / tilesM, tm = blockIdx.x % tilesM;
+    const double* A = g.A + bi * g.strideA + (i64)tm * AT * g.lda;
+    const double* B = g.B + bi * g.strideB;
+    double* C = g.C + bi * g.strideC;
+    ad4 acc[MAXN][2][2];
+#pragma unroll
+    for (int t = 0; t < MAXN; ++t)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[t][i][0] = acc[t][i][1] = ad4{0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < g.K; t0 += AKT) {
+        stage_rowk(As, A + t0, g.lda, tid);
+#pragma unroll
+        for (int tn = 0; tn < MAXN; ++tn) {
+            if (tn < g.tilesN) {
+                stage_km(Bs, B + (i64)t0 * g.ldb + tn * AT, g.ldb, tid);
+                __syncthreads();
+                mma_stage(acc[tn], As, Bs, wave, lane);
+                __syncthreads();
+            }
+        }
+    }
+    const int r0 = tm * AT;
+#pragma unroll
+    for (int tn = 0; tn < MAXN; ++tn) {
+        if (tn < g.tilesN)
+            for_each_out(acc[tn], wave, lane,
+                         [&](int r, int c, double v) { C[(i64)(r0 + r) * g.ldc + tn * AT + c] = v; });
+    }
+}
+
 // Trailing-adjoint product of reverse step k, one output tile of one panel
 // block I and one chunk of block columns J in [Ja, Jb):
 //   W[chunk][I - k - 1] = sum_J Sfull_IJ P_J,  Sfull_IJ = S_IJ (J < I),
@@ -550,7 +590,7 @@ inline int bulk_chunks(int m, int stiles) {
 
 // Workspace layout (doubles) of one gradient evaluation.
 struct AdjWork {
-    i64 dinv, v, z, zs, nll, parts, grad, wp, wp_half, wq, wg, t1, t2, t3, total;
+    i64 dinv, v, z, zs, nll, parts, grad, wp, wp_half, wg, t1, t2, t3, total;
 };
 
 AdjWork adj_layout(int n, int b) {
@@ -588,7 +628,6 @@ AdjWork adj_layout(int n, int b) {
     w.grad = take(4);
     w.wp = take(std::max<i64>(wp, 1));
     w.wp_half = wpHalf;
-    w.wq = take((i64)std::max(1, n - 1) * bb);
     w.wg = take((i64)std::max(1, n - 1) * bb);
     w.t1 = take(bb);
     w.t2 = take(bb);
@@ -656,9 +695,9 @@ int reverse_tail(hs_handle_t h, const double* L, double* Ab, int b, int n, int k
         adj_sub_partials_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, W, m * bb, m * bb, nch);
         HS_LAUNCH_CHECK();
         // Abar_panel = Pbar D^{-1}
-        GemmArgs g1{Pbar, bb, b, 0, Dlo + (i64)k * bb, 0, b, ws + w.wq, bb, b, b, tilesN, tiles, 0};
-        if ((rc = launch_gemm(h, g1, m, tilesN))) return rc;
-        adj_copy_kernel<<<dim3(grid_for(m * bb)), dim3(256), 0, h->stream>>>(Pbar, ws + w.wq, m * bb);
+        // (in place: one work-group per 64-row strip)
+        GemmArgs g1{Pbar, bb, b, 0, Dlo + (i64)k * bb, 0, b, Pbar, bb, b, b, tilesN, tiles, 0};
+        adj_gemm_strip_kernel<<<dim3((unsigned)((i64)m * tilesN)), dim3(ATHREADS), 0, h->stream>>>(g1);
         HS_LAUNCH_CHECK();
         // Dbar -= tril(Abar_panel^T P): one partial per panel block in a
         // single batched launch, summed in a fixed order
